@@ -1,0 +1,14 @@
+"""MI355X time-step engine for BoundaryLayerDynamics.jl (host package).
+
+The compute path is ``libbld.so`` (HIP kernels for gfx950 behind the C ABI of
+``include/bld.h``); this package mirrors the reference's Julia model API on
+top of it.  Build the library with ``build.build()``.
+"""
+
+from .model import (  # noqa: F401
+    Domain, SmoothWall, RoughWall, FreeSlipBoundary, CustomBoundary, SinusoidalMapping,
+    MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, StaticSmagorinskyModel, SSPRK33,
+    incompressible_flow, closedchannelflow, openchannelflow, Model, evolve, pdsize,
+)
+from . import _lib  # noqa: F401
+from .ldlt import BatchLDLt  # noqa: F401

@@ -1,0 +1,757 @@
+// Device kernels of the SSPRK33 time-step engine (gfx950).
+//
+// Transforms.  A 2D c2r of one layer (src/physical_space.jl:277-295) is done in
+// two passes: along x2 (complex, zero-padded from the K2 resolved modes) into an
+// intermediate [x2][k1] array, then along x1, where two real rows are produced
+// by one complex inverse FFT (z = a + i·b).  The forward r2c with truncation,
+// scaling by 1/(N1·N2), prefactor and accumulation (physical_space.jl:208-246)
+// runs the same passes in the opposite order.  Each block transforms `batch`
+// columns/row-pairs of one (term, layer) job in LDS.
+//
+// Physics.  Pointwise kernels for the rotation-form advection products
+// (momentum_advection.jl:179-206), the static Smagorinsky model with its wall
+// model (static_smagorinsky.jl:129-249), the spectral linear terms
+// (molecular_diffusion.jl:57-72, sources.jl:32-35), and one fused kernel per
+// SSPRK33 stage that combines the stage, builds the pressure right-hand side,
+// runs the batched symmetric Thomas sweep per (k1,k2) column and subtracts the
+// gradient (ODEMethods.jl:147-166, pressure.jl:334-426).
+#include "kernels.hpp"
+
+#include "fft.hpp"
+
+namespace bld {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_entry(const LaunchTable &lt, int b) {
+    int ei = 0;
+    while (ei + 1 < lt.n && b >= lt.e[ei + 1].block_begin) ++ei;
+    return ei;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> czero() {
+    return mk(T(0), T(0));
+}
+
+// Spectral value of a c2r term at (k1, i2) of `layer` (derivatives.jl:404-418,566-568).
+template <typename T>
+__device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int recipe, int sel, int layer, int k1,
+                                              int i2) {
+    const size_t P = size_t(S.K1) * S.K2;
+    const size_t idx = size_t(layer) * P + k1 + size_t(S.K1) * i2;
+    const cplx<T> *u = S.state[sel][0], *v = S.state[sel][1], *w = S.state[sel][2];
+    const T d1 = T(S.D1[k1]), d2 = T(S.D2[i2]);
+    switch (recipe) {
+        case R_U: return u[idx];
+        case R_V: return v[idx];
+        case R_W: return w[idx];
+        case R_D1U: return imul(d1, u[idx]);
+        case R_D2U: return imul(d2, u[idx]);
+        case R_D1V: return imul(d1, v[idx]);
+        case R_D2V: return imul(d2, v[idx]);
+        case R_D1W: return imul(d1, w[idx]);
+        case R_D2W: return imul(d2, w[idx]);
+        case R_VORT1: {  // D2·w - (-α·v¯ + α·v⁺)
+            const T a = T(S.alpha_i[layer + 1]);
+            cplx<T> dz = (-a) * v[idx] + a * v[idx + P];
+            return imul(d2, w[idx]) - dz;
+        }
+        case R_VORT2: {  // (-α·u¯ + α·u⁺) - D1·w
+            const T a = T(S.alpha_i[layer + 1]);
+            cplx<T> dz = (-a) * u[idx] + a * u[idx + P];
+            return dz - imul(d1, w[idx]);
+        }
+        case R_VORT3: return imul(d1, v[idx]) - imul(d2, u[idx]);
+        default: return S.in_field[term][idx];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// c2r, pass 1: inverse FFT along x2 of `batch` k1-columns
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+    const int local = blockIdx.x - E.block_begin;
+    const int job = local / E.blocks_per_job;
+    const int chunk = local - job * E.blocks_per_job;
+    int term = 0;
+    while (job >= S.in_job0[term + 1]) ++term;
+    const int layer = job - S.in_job0[term];
+    const int recipe = S.in_recipe[term];
+    const int B = E.batch, N2 = S.N2;
+    cplx<T> *bufA = smem, *bufB = smem + B * N2;
+    const int k1_0 = chunk * B;
+    const int ncols = min(B, S.nk1 - k1_0);
+    for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
+        const int c = idx / N2, pos = idx - c * N2;
+        cplx<T> val = czero<T>();
+        if (c < ncols) {
+            int i2 = -1;
+            if (pos <= S.k2m)
+                i2 = pos;
+            else if (pos >= N2 - S.k2m)
+                i2 = S.K2 - (N2 - pos);
+            if (i2 >= 0) val = term_value(S, term, recipe, sel, layer, k1_0 + c, i2);
+        }
+        bufA[idx] = val;
+    }
+    __syncthreads();
+    cplx<T> *res = fft_block<T, true>(bufA, bufB, S.p2, B, S.tw + S.p2.tw);
+    cplx<T> *out = S.inter + size_t(job) * N2 * S.nk1;
+    for (int idx = threadIdx.x; idx < ncols * N2; idx += blockDim.x) {
+        const int x2 = idx / ncols, c = idx - x2 * ncols;
+        cplx<T> v = res[c * N2 + x2];
+        if (k1_0 + c == 0) v.y = T(0);  // c2r uses the real part of the k1 = 0 column
+        out[size_t(x2) * S.nk1 + k1_0 + c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// c2r, pass 2: Hermitian completion along x1, two real rows per complex FFT
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+    const int local = blockIdx.x - E.block_begin;
+    const int job = local / E.blocks_per_job;
+    const int chunk = local - job * E.blocks_per_job;
+    int term = 0;
+    while (job >= S.in_job0[term + 1]) ++term;
+    const int layer = job - S.in_job0[term];
+    const int B = E.batch, N1 = S.N1, N2 = S.N2, k1m = S.k1m, nk1 = S.nk1;
+    const int npairs_tot = (N2 + 1) / 2;
+    const int pair0 = chunk * B;
+    const int np = min(B, npairs_tot - pair0);
+    cplx<T> *bufA = smem, *bufB = smem + B * N1;
+    const cplx<T> *in = S.inter + size_t(job) * N2 * nk1;
+    for (int idx = threadIdx.x; idx < B * N1; idx += blockDim.x) {
+        const int p = idx / N1, k = idx - p * N1;
+        cplx<T> z = czero<T>();
+        if (p < np) {
+            const int xa = 2 * (pair0 + p), xb = xa + 1;
+            const bool hb = xb < N2;
+            if (k == 0) {
+                cplx<T> a = in[size_t(xa) * nk1];
+                z = mk(a.x, hb ? in[size_t(xb) * nk1].x : T(0));
+            } else if (2 * k == N1 && k <= k1m) {  // Nyquist: real parts only
+                cplx<T> a = in[size_t(xa) * nk1 + k];
+                z = mk(a.x, hb ? in[size_t(xb) * nk1 + k].x : T(0));
+            } else if (k <= k1m) {
+                cplx<T> a = in[size_t(xa) * nk1 + k];
+                cplx<T> b = hb ? in[size_t(xb) * nk1 + k] : czero<T>();
+                z = mk(a.x - b.y, a.y + b.x);
+            } else if (N1 - k <= k1m) {
+                const int kk = N1 - k;
+                cplx<T> a = in[size_t(xa) * nk1 + kk];
+                cplx<T> b = hb ? in[size_t(xb) * nk1 + kk] : czero<T>();
+                z = mk(a.x + b.y, b.x - a.y);
+            }
+        }
+        bufA[idx] = z;
+    }
+    __syncthreads();
+    cplx<T> *res = fft_block<T, true>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
+    T *out = S.in_phys[term] + size_t(layer) * N1 * N2;
+    for (int idx = threadIdx.x; idx < np * N1; idx += blockDim.x) {
+        const int p = idx / N1, x = idx - p * N1;
+        const int xa = 2 * (pair0 + p);
+        cplx<T> z = res[p * N1 + x];
+        out[size_t(xa) * N1 + x] = z.x;
+        if (xa + 1 < N2) out[size_t(xa + 1) * N1 + x] = z.y;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// r2c, pass 1: forward FFT along x1 of two real rows at once, truncated to k1m
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+    const int local = blockIdx.x - E.block_begin;
+    const int job = local / E.blocks_per_job;
+    const int chunk = local - job * E.blocks_per_job;
+    int src = 0;
+    while (job >= S.src_job0[src + 1]) ++src;
+    const int layer = job - S.src_job0[src];
+    const int B = E.batch, N1 = S.N1, N2 = S.N2, nk1 = S.nk1;
+    const int npairs_tot = (N2 + 1) / 2;
+    const int pair0 = chunk * B;
+    const int np = min(B, npairs_tot - pair0);
+    cplx<T> *bufA = smem, *bufB = smem + B * N1;
+    const T *in = S.src_phys[src] + size_t(layer) * N1 * N2;
+    for (int idx = threadIdx.x; idx < B * N1; idx += blockDim.x) {
+        const int p = idx / N1, x = idx - p * N1;
+        cplx<T> z = czero<T>();
+        if (p < np) {
+            const int xa = 2 * (pair0 + p);
+            z.x = in[size_t(xa) * N1 + x];
+            if (xa + 1 < N2) z.y = in[size_t(xa + 1) * N1 + x];
+        }
+        bufA[idx] = z;
+    }
+    __syncthreads();
+    cplx<T> *res = fft_block<T, false>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
+    cplx<T> *out = S.inter + size_t(job) * N2 * nk1;
+    for (int idx = threadIdx.x; idx < np * nk1; idx += blockDim.x) {
+        const int p = idx / nk1, k = idx - p * nk1;
+        const int xa = 2 * (pair0 + p);
+        const cplx<T> zk = res[p * N1 + k];
+        const cplx<T> zn = res[p * N1 + (k == 0 ? 0 : N1 - k)];
+        out[size_t(xa) * nk1 + k] = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
+        if (xa + 1 < N2) out[size_t(xa + 1) * nk1 + k] = mk(T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// r2c, pass 2: forward FFT along x2, truncation, 1/(N1·N2), prefactors,
+// accumulation into the spectral rate (one block owns each output element)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+    const int local = blockIdx.x - E.block_begin;
+    const int job = local / E.blocks_per_job;
+    const int chunk = local - job * E.blocks_per_job;
+    int tg = 0;
+    while (job >= S.tgt_job0[tg + 1]) ++tg;
+    const int layer = job - S.tgt_job0[tg];
+    const int B = E.batch, N2 = S.N2, nk1 = S.nk1;
+    const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
+    const int nT = nH > 0 ? 2 * B : B;  // G columns, then H columns
+    cplx<T> *bufA = smem, *bufB = smem + nT * N2;
+    const int k1_0 = chunk * B;
+    const int ncols = min(B, nk1 - k1_0);
+    for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
+        const int x2 = idx / B, c = idx - x2 * B;  // c fastest: coalesced over k1
+        cplx<T> g = czero<T>(), h = czero<T>();
+        if (c < ncols) {
+            const int k1 = k1_0 + c;
+            const T d1 = T(S.D1[k1]);
+            for (int s = 0; s < nG; ++s) {
+                const int sj = S.src_job0[S.tgt_G[tg][s]] + layer;
+                cplx<T> v = S.inter[(size_t(sj) * N2 + x2) * nk1 + k1];
+                g = g + (S.tgt_Gd1[tg][s] ? imul(d1, v) : v);
+            }
+            for (int s = 0; s < nH; ++s) {
+                const int sj = S.src_job0[S.tgt_H[tg][s]] + layer;
+                h = h + S.inter[(size_t(sj) * N2 + x2) * nk1 + k1];
+            }
+        }
+        bufA[c * N2 + x2] = g;
+        if (nH > 0) bufA[(B + c) * N2 + x2] = h;
+    }
+    __syncthreads();
+    cplx<T> *res = fft_block<T, false>(bufA, bufB, S.p2, nT, S.tw + S.p2.tw);
+    const T scale = T(1.0 / (double(S.N1) * double(S.N2)));
+    const int nres = 2 * S.k2m + 1;
+    cplx<T> *out = S.tgt_out[tg] + size_t(layer) * S.K1 * S.K2;
+    for (int idx = threadIdx.x; idx < ncols * nres; idx += blockDim.x) {
+        const int r = idx / ncols, c = idx - r * ncols;
+        const int i2 = r <= S.k2m ? r : S.K2 - (nres - r);
+        const int pos = r <= S.k2m ? r : N2 - (nres - r);
+        cplx<T> v = res[c * N2 + pos];
+        if (nH > 0) v = v + imul(T(S.D2[i2]), res[(B + c) * N2 + pos]);
+        const size_t o = size_t(i2) * S.K1 + k1_0 + c;
+        out[o] = out[o] + scale * v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// advection products (momentum_advection.jl:179-206); w = 0 at the walls
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict__ sp) {
+    const Space<T> &S = *sp;
+    const size_t P = size_t(S.N1) * S.N2;
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= P * S.n3) return;
+    const int l = int(gid / P);
+    const size_t x = gid - size_t(l) * P;
+    const T *u = S.in_phys[0], *v = S.in_phys[1], *w = S.in_phys[2];
+    const T *o1 = S.in_phys[3], *o2 = S.in_phys[4], *o3 = S.in_phys[5];
+    T *a1 = S.src_phys[0], *a2 = S.src_phys[1], *a3 = S.src_phys[2];
+    const size_t i = gid;
+    const T ul = u[i], vl = v[i], o3l = o3[i];
+    T p2b = T(0), p2a = T(0), p1b = T(0), p1a = T(0);
+    if (l > 0) {
+        const size_t j = size_t(l - 1) * P + x;
+        p2b = o2[j] * w[j];
+        p1b = o1[j] * w[j];
+    }
+    if (l < S.n3 - 1) {
+        const size_t j = size_t(l) * P + x;
+        p2a = o2[j] * w[j];
+        p1a = o1[j] * w[j];
+        const T un = u[j + P], vn = v[j + P];
+        a3[j] = o2[j] * T(0.5) * (ul + un) - o1[j] * T(0.5) * (vl + vn);
+    }
+    a1[i] = o3l * vl - T(0.5) * (p2b + p2a);
+    a2[i] = T(0.5) * (p1b + p1a) - o3l * ul;
+}
+
+// ---------------------------------------------------------------------------
+// static Smagorinsky model (static_smagorinsky.jl:129-249)
+// terms: 0 u, 1 v, 2 w, 3 u_1, 4 v_2, 5 u_2, 6 v_1, 7 w_1, 8 w_2
+// sources: 0 R1 (vel1), 1 R2 (vel2), 2 R3 (vel3), 3 T11 (vel1_1), 4 T22 (vel2_2),
+//          5 T12 (vel1_2 & vel2_1), 6 T13 (vel3_1), 7 T23 (vel3_2)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SgsPoint {
+    const Space<T> &S;
+    size_t P, x;
+    __device__ SgsPoint(const Space<T> &s, size_t p, size_t xx) : S(s), P(p), x(xx) {}
+    __device__ T f(int t, int l) const { return S.in_phys[t][size_t(l) * P + x]; }
+    // S33 on C-layer l (vel3_3 via dx3_i2c!, physical Dirichlet values)
+    __device__ T s33(int l) const {
+        const T a = T(S.alpha_c[l]);
+        const T wb = l > 0 ? f(2, l - 1) : T(S.wbc[0]);
+        const T wa = l < S.n3 - 1 ? f(2, l) : T(S.wbc[1]);
+        return (-a) * wb + a * wa;
+    }
+    __device__ T s12(int l) const { return (f(5, l) + f(6, l)) / T(2); }
+    // S13, S23 on I-layer j (vel1_3 via dx3_c2i!)
+    __device__ T s13(int j) const {
+        const T a = T(S.alpha_i[j + 1]);
+        return ((-a) * f(0, j) + a * f(0, j + 1) + f(7, j)) / T(2);
+    }
+    __device__ T s23(int j) const {
+        const T a = T(S.alpha_i[j + 1]);
+        return ((-a) * f(1, j) + a * f(1, j + 1) + f(8, j)) / T(2);
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restrict__ sp) {
+    const Space<T> &S = *sp;
+    const size_t P = size_t(S.N1) * S.N2;
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= P * S.n3) return;
+    const int l = int(gid / P);
+    const size_t x = gid - size_t(l) * P;
+    SgsPoint<T> q(S, P, x);
+    const int n3 = S.n3;
+    // C-nodes: total_strain! (:208, :225-229)
+    const T S11 = q.f(3, l), S22 = q.f(4, l), S33 = q.s33(l), S12 = q.s12(l);
+    T sc = S11 * S11 + S22 * S22 + S33 * S33 + T(2) * S12 * S12;
+    for (int comp = 0; comp < 2; ++comp) {
+        // lower neighbour (I-layer l-1 or wall), upper neighbour (I-layer l or wall)
+        const T sb = l > 0 ? (comp == 0 ? q.s13(l - 1) : q.s23(l - 1)) : T(0);
+        const T sa = l < n3 - 1 ? (comp == 0 ? q.s13(l) : q.s23(l)) : T(0);
+        if (l == 0 && S.wall_rough[0]) {
+            const T val = q.f(comp, 0) * T(S.fstrain[0]);
+            sc += T(2) * (val * val);
+        } else if (l == n3 - 1 && S.wall_rough[1]) {
+            const T val = q.f(comp, n3 - 1) * T(S.fstrain[1]);
+            sc += T(2) * (val * val);
+        } else {
+            const T s = sb + sa;
+            sc += T(2) * (s * s) / T(4);
+        }
+    }
+    sc = sqrt(T(2) * sc);
+    const T lcl = T(S.lc[l]);
+    S.visc_c[gid] = sc * (lcl * lcl);
+    // I-nodes (:209-220)
+    if (l < n3 - 1) {
+        const T a13 = q.s13(l), a23 = q.s23(l);
+        T si = T(2) * a13 * a13 + T(2) * a23 * a23;
+        T s;
+        s = q.f(3, l) + q.f(3, l + 1);
+        si += s * s / T(4);
+        s = q.f(4, l) + q.f(4, l + 1);
+        si += s * s / T(4);
+        s = S33 + q.s33(l + 1);
+        si += s * s / T(4);
+        s = S12 + q.s12(l + 1);
+        si += T(2) * (s * s) / T(4);
+        si = sqrt(T(2) * si);
+        const T lil = T(S.li[l]);
+        S.visc_i[size_t(l) * P + x] = si * (lil * lil);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ sp) {
+    const Space<T> &S = *sp;
+    const size_t P = size_t(S.N1) * S.N2;
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= P * S.n3) return;
+    const int l = int(gid / P);
+    const size_t x = gid - size_t(l) * P;
+    SgsPoint<T> q(S, P, x);
+    const int n3 = S.n3;
+    const T evc = S.visc_c[gid];
+    // τ11, τ22, τ12 on C-nodes: rates vel1_1 -= τ11, vel2_2 -= τ22, vel1_2/vel2_1 -= τ12
+    S.src_phys[3][gid] = -(T(-2) * evc * q.f(3, l));
+    S.src_phys[4][gid] = -(T(-2) * evc * q.f(4, l));
+    S.src_phys[5][gid] = -(T(-2) * evc * q.s12(l));
+    // vel3 -= d/dx3 τ33 (dx3_c2i!) on I-layer l
+    if (l < n3 - 1) {
+        const T a = T(S.alpha_i[l + 1]);
+        const T t0 = T(-2) * evc * q.s33(l);
+        const T t1 = T(-2) * S.visc_c[gid + P] * q.s33(l + 1);
+        S.src_phys[2][size_t(l) * P + x] = -((-a) * t0 + a * t1);
+        const T evi = S.visc_i[size_t(l) * P + x];
+        S.src_phys[6][size_t(l) * P + x] = -(T(-2) * evi * q.s13(l));
+        S.src_phys[7][size_t(l) * P + x] = -(T(-2) * evi * q.s23(l));
+    }
+    // vel1 -= d/dx3 τ13, vel2 -= d/dx3 τ23 on C-layer l (dx3_i2c! with wall stress)
+    const T ac = T(S.alpha_c[l]);
+    for (int comp = 0; comp < 2; ++comp) {
+        T tb, ta;
+        if (l > 0) {
+            const T evi = S.visc_i[size_t(l - 1) * P + x];
+            tb = T(-2) * evi * (comp == 0 ? q.s13(l - 1) : q.s23(l - 1));
+        } else if (S.wall_rough[0]) {
+            const T u0 = q.f(0, 0), v0 = q.f(1, 0);
+            tb = T(S.fstress[0]) * q.f(comp, 0) * sqrt(u0 * u0 + v0 * v0);
+        } else {
+            tb = T(0);
+        }
+        if (l < n3 - 1) {
+            const T evi = S.visc_i[size_t(l) * P + x];
+            ta = T(-2) * evi * (comp == 0 ? q.s13(l) : q.s23(l));
+        } else if (S.wall_rough[1]) {
+            const T u1 = q.f(0, n3 - 1), v1 = q.f(1, n3 - 1);
+            ta = T(S.fstress[1]) * q.f(comp, n3 - 1) * sqrt(u1 * u1 + v1 * v1);
+        } else {
+            ta = T(0);
+        }
+        S.src_phys[comp][gid] = -((-ac) * tb + ac * ta);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// linear terms: du = ν·Laplacian(u) + S (molecular_diffusion.jl:57-72, sources.jl:32-35)
+// ---------------------------------------------------------------------------
+template <typename T, typename A>
+__device__ __forceinline__ cplx<A> laplacian_point(const LinDesc<T> &L, int f, int l, int m, const cplx<A> &q,
+                                                   const cplx<A> &lo, const cplx<A> &hi) {
+    const int k1 = m % L.K1, i2 = m / L.K1;
+    const A dd = A(L.DD1[k1]) + A(L.DD2[i2]);
+    A a0, a1, a2;
+    if (f < 2) {
+        a0 = A(L.alpha_i[l]);
+        a1 = A(L.alpha_c[l]);
+        a2 = A(L.alpha_i[l + 1]);
+    } else {
+        a0 = A(L.alpha_c[l]);
+        a1 = A(L.alpha_i[l + 1]);
+        a2 = A(L.alpha_c[l + 1]);
+    }
+    const A nu = A(L.nu[f]);
+    const A c0 = a0 * a1, c2 = a1 * a2;
+    const A c1 = dd - a0 * a1 - a1 * a2;
+    return mk(nu * (c0 * lo.x + c1 * q.x + c2 * hi.x), nu * (c0 * lo.y + c1 * q.y + c2 * hi.y));
+}
+
+// ghost layer of a C-node field from 4th-order extrapolation (boundary_conditions.jl:286-356)
+template <typename A>
+__device__ __forceinline__ cplx<A> ghost(int kind, double value, double difference, bool mean, bool upper,
+                                        cplx<A> l1, cplx<A> l2, cplx<A> l3) {
+    cplx<A> g;
+    if (kind == 0) {
+        g = mk((A(-15) * l1.x + A(5) * l2.x - l3.x) / A(5), (A(-15) * l1.y + A(5) * l2.y - l3.y) / A(5));
+        if (mean) g.x += A(16) * A(value) / A(5);
+    } else {
+        g = mk((A(21) * l1.x + A(3) * l2.x - l3.x) / A(23), (A(21) * l1.y + A(3) * l2.y - l3.y) / A(23));
+        const A l0 = upper ? -A(difference) : A(difference);
+        if (mean) g.x -= A(24) * l0 / A(23);
+    }
+    return g;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_linear(LinDesc<T> L, int sel) {
+    const int nk = L.nk, n3 = L.n3;
+    const size_t per_c = size_t(nk) * n3, per_i = size_t(nk) * (n3 - 1);
+    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    int f;
+    size_t r;
+    if (gid < per_c) {
+        f = 0;
+        r = gid;
+    } else if (gid < 2 * per_c) {
+        f = 1;
+        r = gid - per_c;
+    } else if (gid < 2 * per_c + per_i) {
+        f = 2;
+        r = gid - 2 * per_c;
+    } else {
+        return;
+    }
+    const int l = int(r / nk), m = int(r - size_t(l) * nk);
+    const int nl = f < 2 ? n3 : n3 - 1;
+    const cplx<T> *q = L.state[sel][f];
+    cplx<T> *du = L.du[f];
+    const bool mixed00 = (m == 0) && (L.shadow[sel][f] != nullptr);
+    if (mixed00) {
+        // (0,0) mode of the mixed form: float64 from the float64 shadow
+        const cplx<double> *qs = L.shadow[sel][f];
+        cplx<double> res = mk(0.0, 0.0);
+        if (L.has_diff[f]) {
+            cplx<double> lo, hi, c = qs[l];
+            if (f < 2) {
+                lo = l > 0 ? qs[l - 1] : ghost<double>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], true, false, qs[0], qs[1], qs[2]);
+                hi = l < nl - 1 ? qs[l + 1]
+                                : ghost<double>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], true, true, qs[nl - 1], qs[nl - 2], qs[nl - 3]);
+            } else {
+                lo = l > 0 ? qs[l - 1] : mk(L.bc_val[2][0], 0.0);
+                hi = l < nl - 1 ? qs[l + 1] : mk(L.bc_val[2][1], 0.0);
+            }
+            res = laplacian_point<T, double>(L, f, l, m, c, lo, hi);
+        }
+        res.x += L.src[f];
+        L.du00[f][l] = res;
+        du[size_t(l) * nk] = mk(T(0), T(0));
+        return;
+    }
+    cplx<T> res = mk(T(0), T(0));
+    if (L.has_diff[f]) {
+        const size_t i = size_t(l) * nk + m;
+        const cplx<T> c = q[i];
+        cplx<T> lo, hi;
+        if (f < 2) {
+            lo = l > 0 ? q[i - nk]
+                       : ghost<T>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], m == 0, false, q[m], q[nk + m], q[2 * nk + m]);
+            hi = l < nl - 1 ? q[i + nk]
+                            : ghost<T>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], m == 0, true, q[i], q[i - nk], q[i - 2 * nk]);
+        } else {
+            // I-nodes: Dirichlet values enter every mode (boundary_conditions.jl:263-274)
+            lo = l > 0 ? q[i - nk] : mk(T(L.bc_val[2][0]), T(0));
+            hi = l < nl - 1 ? q[i + nk] : mk(T(L.bc_val[2][1]), T(0));
+        }
+        res = laplacian_point<T, T>(L, f, l, m, c, lo, hi);
+    }
+    if (m == 0) res.x += T(L.src[f]);
+    du[size_t(l) * nk + m] = res;
+}
+
+// ---------------------------------------------------------------------------
+// SSPRK33 stage combination + pressure projection, one thread per (k1,k2)
+// ---------------------------------------------------------------------------
+template <typename T, typename A>
+__device__ __forceinline__ cplx<A> combine(int stage, cplx<A> u, cplx<A> ut, cplx<A> du, A dt) {
+    switch (stage) {
+        case 1: return mk(u.x + dt * du.x, u.y + dt * du.y);  // utmp = u + dt·du
+        case 2:
+            return mk((A(3) * u.x + ut.x + dt * du.x) / A(4), (A(3) * u.y + ut.y + dt * du.y) / A(4));
+        case 3:
+            return mk((u.x + A(2) * ut.x + A(2) * dt * du.x) / A(3), (u.y + A(2) * ut.y + A(2) * dt * du.y) / A(3));
+        default: return u;  // projection of the current state only
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, double dt) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= D.nk) return;
+    const int nk = D.nk, n3 = D.n3;
+    const bool mixed00 = (m == 0) && (D.u00[0] != nullptr);
+    // input/output selection: stage 0 projects u in place, 1-2 write utmp, 3 writes u
+    cplx<T> **outp = (stage == 1 || stage == 2) ? D.ut : D.u;
+    cplx<double> **outs = (stage == 1 || stage == 2) ? D.ut00 : D.u00;
+    bool bad = false;
+    auto fin = [&](cplx<T> v) {
+        if (!(isfinite(v.x) && isfinite(v.y))) bad = true;
+    };
+    auto comb = [&](int f, int l) -> cplx<double> {
+        const size_t i = size_t(l) * nk + m;
+        if (mixed00) {
+            cplx<double> du = mk(0.0, 0.0);
+            if (stage > 0) {
+                const cplx<T> dn = D.du[f][i];
+                fin(dn);
+                du = mk(D.du00[f][l].x + double(dn.x), D.du00[f][l].y + double(dn.y));
+                if (!(isfinite(du.x) && isfinite(du.y))) bad = true;
+            }
+            return combine<T, double>(stage, D.u00[f][l], D.ut00[f][l], du, dt);
+        }
+        const cplx<T> u = D.u[f][i];
+        cplx<T> ut = mk(T(0), T(0)), du = mk(T(0), T(0));
+        if (stage == 2 || stage == 3) ut = D.ut[f][i];
+        if (stage > 0) {
+            du = D.du[f][i];
+            fin(du);
+        }
+        const cplx<T> c = combine<T, T>(stage, u, ut, du, T(dt));
+        return mk(double(c.x), double(c.y));
+    };
+    auto store = [&](int f, int l, cplx<double> v) {
+        const size_t i = size_t(l) * nk + m;
+        outp[f][i] = mk(T(v.x), T(v.y));
+        if (mixed00) outs[f][l] = v;
+    };
+    auto load_out = [&](int f, int l) -> cplx<double> {
+        if (mixed00) return outs[f][l];
+        const cplx<T> v = outp[f][size_t(l) * nk + m];
+        return mk(double(v.x), double(v.y));
+    };
+
+    if (!D.has_pressure) {
+        if (stage > 0) {
+            for (int l = 0; l < n3; ++l) {
+                store(0, l, comb(0, l));
+                store(1, l, comb(1, l));
+                if (l < n3 - 1) store(2, l, comb(2, l));
+            }
+        }
+        if (bad) *D.flag = 1;
+        return;
+    }
+    const int k1 = m % D.K1, i2 = m / D.K1;
+    const double d1 = D.D1[k1], d2 = D.D2[i2];
+    // forward sweep: combine, divergence (div!, pressure.jl:368-378), elimination (:126-132)
+    cplx<double> wprev = mk(0.0, 0.0), yprev = mk(0.0, 0.0);
+    for (int l = 0; l < n3; ++l) {
+        const cplx<double> c1 = comb(0, l), c2 = comb(1, l);
+        store(0, l, c1);
+        store(1, l, c2);
+        cplx<double> c3 = mk(0.0, 0.0);
+        if (l < n3 - 1) {
+            c3 = comb(2, l);
+            store(2, l, c3);
+        }
+        const double hf = 1.0 / D.alpha_c[l];
+        // u1·D1·hf + u2·D2·hf with D = i·d
+        cplx<double> r = mk((-d1 * c1.y) * hf + (-d2 * c2.y) * hf, (d1 * c1.x) * hf + (d2 * c2.x) * hf);
+        if (l > 0) {
+            r = mk(r.x - wprev.x, r.y - wprev.y);
+        }
+        if (l < n3 - 1) {
+            r = mk(r.x + c3.x, r.y + c3.y);
+        }
+        if (m == 0) {
+            if (l == 0) r.x -= D.wbc[0];
+            if (l == n3 - 1) r.x += D.wbc[1];
+            if (l == n3 - 1) r = mk(0.0, 0.0);  // adjust_pressure_rhs! (:396)
+        }
+        const double g = D.gamma[size_t(l) * nk + m];
+        cplx<double> y;
+        if (l == 0) {
+            y = mk(g * r.x, g * r.y);
+        } else {
+            const double b = D.alpha_i[l];
+            y = mk(g * (r.x - b * yprev.x), g * (r.y - b * yprev.y));
+        }
+        D.pbuf[size_t(l) * nk + m] = y;
+        yprev = y;
+        wprev = c3;
+    }
+    // backward sweep (:155-160) and gradient (add_gradient!, :405-426)
+    cplx<double> pnext = mk(0.0, 0.0);
+    for (int l = n3 - 1; l >= 0; --l) {
+        cplx<double> p = D.pbuf[size_t(l) * nk + m];
+        if (l < n3 - 1) {
+            const double gb = D.gamma[size_t(l) * nk + m] * D.alpha_i[l + 1];
+            p = mk(p.x - gb * pnext.x, p.y - gb * pnext.y);
+        }
+        cplx<double> v1 = load_out(0, l), v2 = load_out(1, l);
+        // f += (-1·D)·p with D = i·d: (-i d)·p = (d·p.y, -d·p.x)
+        v1 = mk(v1.x + d1 * p.y, v1.y - d1 * p.x);
+        v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
+        store(0, l, v1);
+        store(1, l, v2);
+        if (l < n3 - 1) {
+            const double a = D.alpha_i[l + 1];
+            cplx<double> v3 = load_out(2, l);
+            const cplx<double> g = mk((-a) * p.x + a * pnext.x, (-a) * p.y + a * pnext.y);
+            v3 = mk(v3.x - g.x, v3.y - g.y);
+            store(2, l, v3);
+        }
+        pnext = p;
+    }
+    if (bad) *D.flag = 1;
+}
+
+// finite check of the rates (BoundaryLayerDynamics.jl:155-157)
+template <typename T>
+__global__ void __launch_bounds__(256) k_check_finite(const cplx<T> *a, size_t n, int *flag) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const cplx<T> v = a[i];
+        if (!(isfinite(v.x) && isfinite(v.y))) *flag = 1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fill(cplx<T> *a, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        a[i] = mk(T(0), T(0));
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel,
+                      hipStream_t st) {
+    if (nblocks <= 0) return;
+    switch (kind) {
+        case PASS_C2R_X2: k_c2r_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt, sel); break;
+        case PASS_C2R_X1: k_c2r_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
+        case PASS_R2C_X1: k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
+        case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
+    }
+}
+
+template <typename T>
+void launch_physics(int kind, const Space<T> *space_dev, size_t npoints, hipStream_t st) {
+    const int nb = int((npoints + 255) / 256);
+    switch (kind) {
+        case PHYS_ADV: k_adv_products<T><<<nb, 256, 0, st>>>(space_dev); break;
+        case PHYS_SGS_VISC: k_sgs_viscosity<T><<<nb, 256, 0, st>>>(space_dev); break;
+        case PHYS_SGS_STRESS: k_sgs_stress<T><<<nb, 256, 0, st>>>(space_dev); break;
+    }
+}
+
+template <typename T>
+void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st) {
+    const size_t n = size_t(L.nk) * (3 * size_t(L.n3) - 1);
+    k_linear<T><<<int((n + 255) / 256), 256, 0, st>>>(L, sel);
+}
+
+template <typename T>
+void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st) {
+    k_stage_project<T><<<(D.nk + 63) / 64, 64, 0, st>>>(D, stage, dt);
+}
+
+template <typename T>
+void launch_check_finite(const void *a, size_t n, int *flag, hipStream_t st) {
+    int nb = int(std::min<size_t>((n + 255) / 256, 4096));
+    if (nb > 0) k_check_finite<T><<<nb, 256, 0, st>>>(reinterpret_cast<const cplx<T> *>(a), n, flag);
+}
+
+template <typename T>
+void launch_fill(void *a, size_t n, hipStream_t st) {
+    int nb = int(std::min<size_t>((n + 255) / 256, 4096));
+    if (nb > 0) k_fill<T><<<nb, 256, 0, st>>>(reinterpret_cast<cplx<T> *>(a), n);
+}
+
+#define BLD_INSTANTIATE(T)                                                                                         \
+    template void launch_transform<T>(int, const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
+    template void launch_physics<T>(int, const Space<T> *, size_t, hipStream_t);                                  \
+    template void launch_linear<T>(const LinDesc<T> &, int, hipStream_t);                                         \
+    template void launch_stage_project<T>(const ProjDesc<T> &, int, double, hipStream_t);                         \
+    template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \
+    template void launch_fill<T>(void *, size_t, hipStream_t);
+
+BLD_INSTANTIATE(float)
+BLD_INSTANTIATE(double)
+
+}  // namespace bld

@@ -1,0 +1,58 @@
+// Descriptors and host launchers of the engine kernels.
+#pragma once
+
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace bld {
+
+enum PassKind : int { PASS_C2R_X2 = 0, PASS_C2R_X1 = 1, PASS_R2C_X1 = 2, PASS_R2C_X2 = 3 };
+enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2 };
+
+// Spectral linear terms of one model instance.
+template <typename T>
+struct LinDesc {
+    int K1, K2, nk, n3;
+    const double *DD1, *DD2;            // -(k·2π/L)^2
+    const double *alpha_c, *alpha_i;    // [n3], [n3+1]
+    int has_diff[3];
+    double nu[3], src[3];
+    int bc_kind[3][2];
+    double bc_val[3][2], bc_diff[3][2];
+    const cplx<T> *state[2][3];         // input state: 0 = u, 1 = utmp
+    const cplx<double> *shadow[2][3];   // float64 (0,0) shadows (mixed form) or null
+    cplx<T> *du[3];
+    cplx<double> *du00[3];              // float64 linear rate of the (0,0) mode (mixed form)
+};
+
+// Stage combination + projection of one model instance.
+template <typename T>
+struct ProjDesc {
+    int K1, K2, nk, n3;
+    int has_pressure;
+    const double *D1, *D2;
+    const double *alpha_c, *alpha_i;
+    const double *gamma;                // [n3][nk] LDLt factor (pressure.jl:86-95)
+    double wbc[2];                      // vel3 Dirichlet values
+    cplx<double> *pbuf;                 // [n3][nk] scratch
+    cplx<T> *u[3], *ut[3], *du[3];
+    cplx<double> *u00[3], *ut00[3], *du00[3];
+    int *flag;
+};
+
+template <typename T>
+void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel,
+                      hipStream_t st);
+template <typename T>
+void launch_physics(int kind, const Space<T> *space_dev, size_t npoints, hipStream_t st);
+template <typename T>
+void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st);
+template <typename T>
+void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st);
+template <typename T>
+void launch_check_finite(const void *a, size_t n, int *flag, hipStream_t st);
+template <typename T>
+void launch_fill(void *a, size_t n, hipStream_t st);
+
+}  // namespace bld

@@ -1,0 +1,500 @@
+"""Host-side mirror of the BoundaryLayerDynamics.jl model API over the HIP engine.
+
+The reference builds a `Model` from a resolution, a `Domain` and a process list
+(src/BoundaryLayerDynamics.jl:72-97) and advances it with `evolve!` (:190-219).
+This module keeps those names and meanings; the discretisation constants that
+the reference's `init_process` methods compute are computed here (float64, on
+the host, once) and handed to the engine through `bld_config`.  Everything that
+runs per time step runs on the GPU.
+
+Arrays use the reference layout: spectral fields are complex128 Fortran-ordered
+arrays of shape (k1max+1, 2*k2max+1, n3) (vel3: n3-1 layers), physical fields
+are float64 Fortran-ordered (N1, N2, layers).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import BldConfig, BldError, dptr
+
+# ---------------------------------------------------------------------------
+# Domain and boundaries (src/Domains.jl)
+# ---------------------------------------------------------------------------
+
+
+class SmoothWall:
+    """No-slip, no-penetration wall (Domains.jl:322)."""
+
+
+class RoughWall:
+    """Rough wall with roughness length z0 (Domains.jl:331-335)."""
+
+    def __init__(self, roughness, von_karman_constant=0.4):
+        self.roughness = roughness
+        self.von_karman_constant = von_karman_constant
+
+
+class FreeSlipBoundary:
+    """No penetration, vanishing tangential gradients (Domains.jl:343)."""
+
+
+class CustomBoundary:
+    """Explicit boundary conditions per field (Domains.jl:360-363): values are
+    'dirichlet', 'neumann', ('dirichlet', value) or ('neumann', gradient)."""
+
+    def __init__(self, **behaviors):
+        self.behaviors = behaviors
+
+
+class SinusoidalMapping:
+    """Grid stretching x3 ∝ sin(ζηπ/2)/sin(ηπ/2) (Domains.jl:234-273)."""
+
+    def __init__(self, eta, variant="auto"):
+        if not 0 < eta < 1:
+            raise ValueError("SinusoidalMapping requires 0 < η < 1")
+        self.parameter = eta
+        self.variant = variant
+
+
+def _has_boundary_layer(b):  # Domains.jl:365-367
+    if isinstance(b, (RoughWall, SmoothWall)):
+        return True
+    if isinstance(b, FreeSlipBoundary):
+        return False
+    raise ValueError("Could not determine variant of SinusoidalMapping")
+
+
+class Domain:
+    """Domains.jl:173-190: periodic in x1, x2; bounded in x3 (optionally mapped)."""
+
+    def __init__(self, dims, lower_boundary, upper_boundary, mapping=None):
+        self.hsize = (float(dims[0]), float(dims[1]))
+        l3 = 1.0 if len(dims) == 2 else dims[2]
+        x3min, x3max = (0.0, float(l3)) if np.isscalar(l3) else (float(l3[0]), float(l3[1]))
+        self.lower_boundary, self.upper_boundary = lower_boundary, upper_boundary
+        if mapping is None:
+            self.vmap = lambda z: x3min + z * (x3max - x3min)
+            self.Dvmap = lambda z: x3max - x3min
+        elif isinstance(mapping, tuple):
+            self.vmap, self.Dvmap = mapping
+        else:
+            eta = float(mapping.parameter)
+            l = x3max - x3min
+            variant = mapping.variant
+            if variant == "auto":
+                bls = (_has_boundary_layer(lower_boundary), _has_boundary_layer(upper_boundary))
+                variant = {(True, True): "symmetric", (True, False): "below", (False, True): "above"}.get(bls)
+                if variant is None:
+                    raise ValueError("Could not determine variant of SinusoidalMapping")
+            s = math.sin(eta * math.pi / 2)
+            shift = {"symmetric": lambda z: 2 * z - 1, "below": lambda z: z - 1, "above": lambda z: z}[variant]
+            if variant == "symmetric":
+                self.vmap = lambda z: x3min + l / 2 * (1 + math.sin(eta * shift(z) * math.pi / 2) / s)
+                self.Dvmap = lambda z: l * eta * math.pi / 2 / s * math.cos(eta * shift(z) * math.pi / 2)
+            elif variant == "below":
+                self.vmap = lambda z: x3min + l * (1 + math.sin(eta * shift(z) * math.pi / 2) / s)
+                self.Dvmap = lambda z: l * eta * math.pi / 2 / s * math.cos(eta * shift(z) * math.pi / 2)
+            else:
+                self.vmap = lambda z: x3min + l * (math.sin(eta * shift(z) * math.pi / 2) / s)
+                self.Dvmap = lambda z: l * eta * math.pi / 2 / s * math.cos(eta * shift(z) * math.pi / 2)
+
+    def size(self, dim):
+        return self.hsize[dim - 1] if dim in (1, 2) else self.vmap(1.0) - self.vmap(0.0)
+
+
+# ---------------------------------------------------------------------------
+# Processes (src/processes/*.jl) and ODE methods (src/ODEMethods.jl)
+# ---------------------------------------------------------------------------
+
+
+class MomentumAdvection:
+    def __init__(self, dealiasing="quadratic"):
+        self.dealiasing = dealiasing
+
+
+class MolecularDiffusion:
+    def __init__(self, field, diffusivity):
+        self.field = field
+        self.diffusivity = diffusivity
+
+
+class Pressure:
+    def __init__(self, batch_size=64):  # batch_size only pipelines MPI ranks in the reference
+        self.batch_size = batch_size
+
+
+class ConstantSource:
+    def __init__(self, field, strength=1.0):
+        self.field = field
+        self.strength = strength
+
+
+class StaticSmagorinskyModel:
+    def __init__(self, dealiasing=None, Cs=0.1, wall_damping=True, wall_damping_exponent=2):
+        self.dealiasing = dealiasing
+        self.model_constant = Cs
+        self.wall_damping = wall_damping
+        self.wall_damping_exponent = wall_damping_exponent
+
+
+class SSPRK33:
+    """Three-stage SSP Runge-Kutta (ODEMethods.jl:50,147-166), the engine's method."""
+
+
+def incompressible_flow(viscosity, sgs_model=None, constant_forcing=None, constant_flux=None):
+    """BoundaryLayerDynamics.jl:352-368."""
+    if constant_flux is not None:
+        raise NotImplementedError("ConstantMean (constant flux) is not part of the GPU step engine")
+    procs = [MomentumAdvection()] + [MolecularDiffusion(v, viscosity) for v in ("vel1", "vel2", "vel3")]
+    procs.append(Pressure())
+    forcing = 1 if constant_forcing is None else constant_forcing
+    forcing = forcing if isinstance(forcing, (tuple, list)) else (forcing,)
+    procs += [ConstantSource(v, f) for v, f in zip(("vel1", "vel2"), forcing)]
+    if sgs_model is not None:
+        procs.append(sgs_model)
+    return procs
+
+
+def closedchannelflow(Re, dims, precision="fp64", **kw):  # BoundaryLayerDynamics.jl:134-138
+    dom = Domain((4 * math.pi, 2 * math.pi, 2), SmoothWall(), SmoothWall())
+    return Model(dims, dom, incompressible_flow(1 / Re, **kw), precision=precision)
+
+
+def openchannelflow(Re, dims, roughness_length=None, precision="fp64", **kw):  # :140-145
+    wall = SmoothWall() if roughness_length is None else RoughWall(roughness_length)
+    dom = Domain((4 * math.pi, 2 * math.pi, 1), wall, FreeSlipBoundary())
+    return Model(dims, dom, incompressible_flow(1 / Re, **kw), precision=precision)
+
+
+# ---------------------------------------------------------------------------
+# grid helpers (src/Grids.jl, src/physical_space.jl)
+# ---------------------------------------------------------------------------
+
+NODES = {"vel1": "C", "vel2": "C", "vel3": "I"}
+FIELD_INDEX = {"vel1": 0, "vel2": 1, "vel3": 2}
+
+
+def pdsize(kmax_pair, dealiasing):
+    """physical_space.jl:42-56."""
+    if dealiasing is None:
+        return tuple(2 + 2 * k for k in kmax_pair)
+    if isinstance(dealiasing, tuple):
+        return dealiasing
+    if dealiasing == "quadratic":
+        return tuple(3 * k + 3 if math.log2(1 + k).is_integer() else 3 * k + (1 if k % 2 else 2)
+                     for k in kmax_pair)
+    raise ValueError(f"Unsupported dealiasing type: {dealiasing}")
+
+
+def _zeta(n3, ns, neighbors=False):  # Grids.jl:130-141
+    if ns == "C":
+        return [j / n3 for j in range(n3 + 1)] if neighbors else [(2 * i - 1) / (2 * n3) for i in range(1, n3 + 1)]
+    return [(2 * i - 1) / (2 * n3) for i in range(1, n3 + 1)] if neighbors else [i / n3 for i in range(1, n3)]
+
+
+def _bctype(boundary, field):  # boundary_conditions.jl:85-95
+    if isinstance(boundary, CustomBoundary):
+        b = boundary.behaviors[field]
+    elif isinstance(boundary, (SmoothWall, RoughWall)):
+        b = "dirichlet"
+    elif isinstance(boundary, FreeSlipBoundary):
+        b = "dirichlet" if field == "vel3" else "neumann"
+    else:
+        raise ValueError(f"unsupported boundary {boundary!r}")
+    kind, value = (b[0], float(b[1])) if isinstance(b, tuple) else (b, 0.0)
+    if kind not in ("dirichlet", "neumann"):
+        raise ValueError(f"unsupported boundary condition {kind!r}")
+    return kind, value
+
+
+# ---------------------------------------------------------------------------
+# Model
+# ---------------------------------------------------------------------------
+
+
+class Discretization:
+    """The constants that the reference's init_process methods compute
+    (src/processes/*.jl), packed into the engine's `bld_config`."""
+
+    def __init__(self, resolution, domain, processes, precision="fp64"):
+        n1, n2, n3 = resolution
+        self.k1max, self.k2max = (n1 - 1) // 2, (n2 - 1) // 2
+        if n3 < 3:
+            raise ValueError("Due to boundary stencils, the grid needs at least 3 vertical layers")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.n3 = n3
+        self.domain = domain
+        flat = []
+        for p in processes:
+            flat.extend(p if isinstance(p, (tuple, list)) else [p])
+        self.processes = flat
+        self.precision = precision
+        self.fields = []
+        for p in flat:  # Processes.jl:326-336
+            fs = ("vel1", "vel2", "vel3") if isinstance(
+                p, (MomentumAdvection, Pressure, StaticSmagorinskyModel)) else (p.field,)
+            for f in fs:
+                if f not in FIELD_INDEX:
+                    raise NotImplementedError(f"field {f!r}: the GPU engine advances vel1, vel2, vel3")
+                if f not in self.fields:
+                    self.fields.append(f)
+        self._keep = []
+        self.config = self._config()
+
+    def _arr(self, values):
+        a = np.array(values, dtype=np.float64)
+        self._keep.append(a)
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def _config(self):
+        """The constants of init_process for each process (src/processes/*.jl)."""
+        d, n3 = self.domain, self.n3
+        alpha = lambda z: n3 / d.Dvmap(z)  # derivatives.jl:10-19
+        c = BldConfig()
+        c.abi_version = _lib.BLD_ABI_VERSION
+        c.precision = {"fp64": _lib.BLD_FP64, "fp32": _lib.BLD_FP32_MIXED}[self.precision]
+        c.k1max, c.k2max, c.n3 = self.k1max, self.k2max, n3
+        c.L1, c.L2 = d.hsize
+        c.alpha_c = self._arr([alpha(z) for z in _zeta(n3, "C")])
+        c.alpha_i = self._arr([alpha(z) for z in _zeta(n3, "C", neighbors=True)])
+        for fi, f in enumerate(("vel1", "vel2", "vel3")):
+            for side, (b, z) in enumerate(((d.lower_boundary, 0.0), (d.upper_boundary, 1.0))):
+                kind, value = _bctype(b, f)
+                bc = c.bc[fi][side]
+                bc.kind = _lib.BLD_BC_DIRICHLET if kind == "dirichlet" else _lib.BLD_BC_NEUMANN
+                bc.value = value
+                bc.difference = value * (d.Dvmap(z) / n3) if kind == "neumann" else 0.0
+        for p in self.processes:
+            if isinstance(p, MomentumAdvection):
+                c.advection = 1
+                c.adv_n1, c.adv_n2 = pdsize((self.k1max, self.k2max), p.dealiasing)
+            elif isinstance(p, MolecularDiffusion):
+                fi = FIELD_INDEX[p.field]
+                if c.diffusion[fi]:
+                    raise NotImplementedError("more than one diffusion process per field")
+                c.diffusion[fi] = 1
+                c.diffusivity[fi] = float(p.diffusivity)
+            elif isinstance(p, Pressure):
+                c.pressure = 1
+            elif isinstance(p, ConstantSource):
+                c.source[FIELD_INDEX[p.field]] += float(p.strength)
+            elif isinstance(p, StaticSmagorinskyModel):
+                self._config_sgs(c, p)
+            else:
+                raise NotImplementedError(f"process {type(p).__name__} is not part of the GPU step engine")
+        return c
+
+    def _config_sgs(self, c, p):
+        """static_smagorinsky.jl:44-123."""
+        d, n3 = self.domain, self.n3
+        c.sgs = 1
+        c.sgs_n1, c.sgs_n2 = pdsize((self.k1max, self.k2max), p.dealiasing)
+        D1 = d.size(1) / (2 * self.k1max + 2)
+        D2 = d.size(2) / (2 * self.k2max + 2)
+        zc, zi = _zeta(n3, "C"), _zeta(n3, "I")
+        lc = [p.model_constant * np.cbrt(D1 * D2 * (d.Dvmap(z) / n3)) for z in zc]
+        li = [p.model_constant * np.cbrt(D1 * D2 * (d.Dvmap(z) / n3)) for z in zi]
+        x3min, x3max = d.vmap(0.0), d.vmap(1.0)
+
+        def mixing(x3, wall):  # :125-127
+            if isinstance(wall, FreeSlipBoundary):
+                return math.inf
+            if isinstance(wall, RoughWall):
+                return x3 * wall.von_karman_constant
+            raise ValueError("SGS model only supports rough-wall and free-slip boundaries")
+
+        if p.wall_damping:
+            N = p.wall_damping_exponent
+            ml = lambda x3: min(mixing(x3 - x3min, d.lower_boundary), mixing(x3max - x3, d.upper_boundary))
+            lc = [(a ** (-N) + ml(d.vmap(z)) ** (-N)) ** (-1 / N) for a, z in zip(lc, zc)]
+            li = [(a ** (-N) + ml(d.vmap(z)) ** (-N)) ** (-1 / N) for a, z in zip(li, zi)]
+        c.sgs_lc = self._arr(lc)
+        c.sgs_li = self._arr(li)
+        dx3 = (d.vmap(zc[0]) - x3min, x3max - d.vmap(zc[-1]))
+        for side, (b, sign) in enumerate(((d.lower_boundary, 1.0), (d.upper_boundary, -1.0))):
+            if isinstance(b, RoughWall):
+                lg = math.log(dx3[side] / b.roughness)
+                c.sgs_wall[side] = _lib.BLD_WALL_ROUGH
+                c.sgs_factor_strain[side] = sign * (1 / (2 * dx3[side] * lg))
+                c.sgs_factor_stress[side] = sign * (-b.von_karman_constant ** 2 / lg ** 2)
+            elif isinstance(b, FreeSlipBoundary):
+                c.sgs_wall[side] = _lib.BLD_WALL_FREESLIP
+            else:
+                raise ValueError("SGS model only supports rough-wall and free-slip boundaries")
+
+
+class Model:
+    """A discretised model on one GPU (BoundaryLayerDynamics.jl:72-97).
+
+    ``precision`` is ``"fp64"`` (plain form) or ``"fp32"`` (float32 fields with
+    float64 pressure elimination, (0,0) mode and forcing)."""
+
+    def __init__(self, resolution, domain, processes, precision="fp64"):
+        disc = Discretization(resolution, domain, processes, precision)
+        self.disc = disc
+        self.k1max, self.k2max, self.n3 = disc.k1max, disc.k2max, disc.n3
+        self.domain, self.processes, self.precision, self.fields = domain, disc.processes, precision, disc.fields
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = lib.bld_create(ctypes.byref(disc.config), ctypes.byref(h))
+        if rc != 0:
+            raise BldError(rc, lib.bld_last_error(None).decode())
+        self._h = h
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.bld_destroy(h)
+            self._h = None
+
+    @property
+    def fdsize(self):
+        return (self.k1max + 1, 2 * self.k2max + 1)
+
+    def nlayers(self, field):
+        return self.n3 if NODES[field] == "C" else self.n3 - 1
+
+    # -- engine calls ------------------------------------------------------
+    def _check(self, rc):
+        if rc != 0:
+            raise BldError(rc, self._lib.bld_last_error(self._h).decode())
+
+    def _empty(self, f):
+        return np.zeros((*self.fdsize, self.nlayers(f)), dtype=np.complex128, order="F")
+
+    def _triple(self, arrays):
+        return [dptr(arrays.get(f)) for f in ("vel1", "vel2", "vel3")]
+
+    @property
+    def state(self):
+        """Copy of the state (model.state of the reference) as host arrays."""
+        out = {f: self._empty(f) for f in ("vel1", "vel2", "vel3")}
+        self._check(self._lib.bld_get_state(self._h, *self._triple(out)))
+        return {f: out[f] for f in self.fields}
+
+    def set_state(self, **fields):
+        arrays = {f: np.asfortranarray(np.asarray(v, dtype=np.complex128)) for f, v in fields.items()}
+        for f, a in arrays.items():
+            if a.shape != (*self.fdsize, self.nlayers(f)):
+                raise ValueError(f"{f}: expected shape {(*self.fdsize, self.nlayers(f))}, got {a.shape}")
+        self._check(self._lib.bld_set_state(self._h, *self._triple(arrays)))
+
+    def rates(self):
+        """compute_rates! of the current state (Processes.jl:39-62)."""
+        out = {f: self._empty(f) for f in ("vel1", "vel2", "vel3")}
+        self._check(self._lib.bld_rates(self._h, *self._triple(out)))
+        return {f: out[f] for f in self.fields}
+
+    def project(self):
+        """apply_projections! (Processes.jl:67-75)."""
+        self._check(self._lib.bld_project(self._h))
+
+    def default_size(self):
+        return pdsize((self.k1max, self.k2max), None)
+
+    def coordinates(self, field, dim):
+        n1, n2 = self.default_size()
+        if dim == 1:
+            return self.domain.hsize[0] * np.arange(n1) / n1
+        if dim == 2:
+            return self.domain.hsize[1] * np.arange(n2) / n2
+        return np.array([self.domain.vmap(z) for z in _zeta(self.n3, NODES[field])])
+
+    def set_physical(self, field, values):
+        a = np.asfortranarray(np.asarray(values, dtype=np.float64))
+        n1, n2 = self.default_size()
+        if a.shape != (n1, n2, self.nlayers(field)):
+            raise ValueError("physical values must have the default physical size")
+        self._check(self._lib.bld_set_field_physical(self._h, FIELD_INDEX[field], dptr(a)))
+
+    def initialize(self, **ics):
+        """State.jl:11-31: evaluate functions (or constants) on the default grid."""
+        for f in ics:
+            if f not in self.fields:
+                raise KeyError(f"Cannot initialize `{f}`: unknown field")
+        n1, n2 = self.default_size()
+        for f in ("vel1", "vel2", "vel3"):
+            fn = ics.get(f, 0.0)
+            X = np.meshgrid(self.coordinates(f, 1), self.coordinates(f, 2), self.coordinates(f, 3), indexing="ij")
+            if np.isscalar(fn):
+                vals = np.full(X[0].shape, float(fn))
+            else:
+                vals = _evaluate(fn, *X)
+            self.set_physical(f, vals)
+
+    def __getitem__(self, field):
+        """Default-size physical values (State.jl:81-85)."""
+        n1, n2 = self.default_size()
+        out = np.zeros((n1, n2, self.nlayers(field)), dtype=np.float64, order="F")
+        self._check(self._lib.bld_get_field_physical(self._h, FIELD_INDEX[field], dptr(out)))
+        return out
+
+    # -- time integration ----------------------------------------------------
+    def init_problem(self, t1=0.0):
+        self._check(self._lib.bld_init_problem(self._h, t1))
+
+    def step(self, dt, nsteps, t=0.0, t_end=0.0, graph=True):
+        tt = ctypes.c_double(t)
+        fn = self._lib.bld_step_graph if graph else self._lib.bld_step
+        self._check(fn(self._h, dt, nsteps, ctypes.byref(tt), t_end))
+        return tt.value
+
+    def time_steps(self, dt, nsteps, graph=True):
+        ms = ctypes.c_double()
+        self._check(self._lib.bld_time_steps(self._h, dt, nsteps, 1 if graph else 0, ctypes.byref(ms)))
+        return ms.value
+
+    def set_profiling(self, on):
+        self._check(self._lib.bld_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        out = {}
+        for k, name in enumerate(_lib.KERNEL_KINDS):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            self._check(self._lib.bld_kernel_times(self._h, k, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+
+def _evaluate(fn, X1, X2, X3):
+    try:
+        v = np.asarray(fn(X1, X2, X3), dtype=float)
+        if v.shape == X1.shape:
+            return v
+    except (TypeError, ValueError):
+        pass
+    out = np.empty(X1.shape)
+    for idx in np.ndindex(X1.shape):
+        out[idx] = fn(X1[idx], X2[idx], X3[idx])
+    return out
+
+
+def _approxdiv(a, b):  # util.jl:19-24
+    r = a / b
+    n = int(round(r))
+    if not (r == n or abs(r - n) <= math.sqrt(np.finfo(float).eps) * max(abs(r), abs(n))):
+        raise ValueError("Division has a non-integer result")
+    return n
+
+
+def evolve(model, tspan, dt, method=None, graph=True):
+    """evolve! (BoundaryLayerDynamics.jl:190-219) without output modules.
+
+    The state is projected and its rate evaluated at t1 (ODEProblem,
+    ODEMethods.jl:179-185), then nt SSPRK33 steps run on the GPU, each followed
+    by the rate evaluation of ODEMethods.jl:231.  Returns the final time."""
+    if method is not None and not isinstance(method, SSPRK33):
+        raise NotImplementedError("the GPU engine integrates with SSPRK33")
+    t1, t2 = (0.0, float(tspan)) if np.isscalar(tspan) else (float(tspan[0]), float(tspan[-1]))
+    if not t1 <= t2:
+        raise ValueError("Integration start time is larger than end time")
+    model.init_problem(t1)
+    if t2 > t1:
+        nt = _approxdiv(t2 - t1, dt)
+        return model.step(dt, nt, t=t1, t_end=t2, graph=graph)
+    return t1

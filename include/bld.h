@@ -1,0 +1,164 @@
+/*
+ * bld.h — C ABI of the MI355X time-step engine for BoundaryLayerDynamics.jl.
+ *
+ * The engine advances the reference's incompressible-flow model (Fourier in
+ * x1/x2, staggered finite differences in x3) by SSPRK33 steps on one GPU.
+ * It replaces, for one step, the Julia closures `rate!` / `projection!` that
+ * the reference builds in src/BoundaryLayerDynamics.jl:149-165 and consumes
+ * in src/ODEMethods.jl:147-166,217-234.
+ *
+ * Conventions
+ *  - Plain C types only; every function returns 0 (BLD_OK) or a negative
+ *    bld_status, and the message is available from bld_last_error().
+ *  - Spectral fields are complex float64, interleaved re/im, in the Julia
+ *    column-major layout of src/Grids.jl:91-93: element (i1, i2, i3) at
+ *    index i1 + K1*(i2 + K2*i3) with K1 = k1max+1, K2 = 2*k2max+1, i2 ordered
+ *    as k2 = [0..k2max, -k2max..-1] (src/Grids.jl:74-83).  vel1/vel2 have n3
+ *    layers (C-nodes), vel3 has n3-1 layers (I-nodes).
+ *  - The engine owns all device memory; it never keeps a host pointer after
+ *    a call returns.  One engine per host thread; calls are synchronous.
+ */
+#ifndef BLD_H
+#define BLD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BLD_ABI_VERSION 1
+
+typedef enum bld_status {
+    BLD_OK = 0,
+    BLD_EARG = -1,          /* invalid argument (sizes, pointers, config) */
+    BLD_EUNSUPPORTED = -2,  /* configuration the reference rejects */
+    BLD_EHIP = -3,          /* HIP runtime error */
+    BLD_EDIVERGED = -4,     /* "The simulation has diverged." */
+    BLD_ENOMEM = -5
+} bld_status;
+
+typedef enum bld_precision {
+    BLD_FP64 = 0,           /* plain form: everything in float64 */
+    BLD_FP32_MIXED = 1      /* float32 fields; float64 pressure solve, (0,0) mode and forcing */
+} bld_precision;
+
+typedef enum bld_bc_kind { BLD_BC_DIRICHLET = 0, BLD_BC_NEUMANN = 1 } bld_bc_kind;
+typedef enum bld_wall_kind { BLD_WALL_FREESLIP = 0, BLD_WALL_ROUGH = 1 } bld_wall_kind;
+
+/* Boundary condition of one velocity component at one wall
+ * (ConstantValue / ConstantGradient of src/boundary_conditions.jl:36-43,53-57):
+ * Dirichlet uses `value`; Neumann uses `difference` = gradient * (dx3/dζ)/n3. */
+typedef struct bld_bc {
+    int32_t kind;           /* bld_bc_kind */
+    int32_t reserved;
+    double value;
+    double difference;
+} bld_bc;
+
+/* Discretised processes of one Model (src/BoundaryLayerDynamics.jl:72-97).
+ * Every constant is the one the reference's init_process computes, so the GPU
+ * never re-derives physics. */
+typedef struct bld_config {
+    int32_t abi_version;    /* BLD_ABI_VERSION */
+    int32_t precision;      /* bld_precision */
+    int32_t k1max, k2max, n3;
+    int32_t reserved0;
+    double L1, L2;          /* horizontal domain size */
+    const double *alpha_c;  /* [n3]   α = n3/(dx3/dζ) at C-nodes ζ=(i-1/2)/n3 (derivatives.jl:10-19) */
+    const double *alpha_i;  /* [n3+1] α at I-nodes ζ=i/n3, i = 0..n3 (walls included) */
+    bld_bc bc[3][2];        /* [vel1,vel2,vel3][lower,upper] */
+    /* MolecularDiffusion (molecular_diffusion.jl:31-72), per field */
+    int32_t diffusion[3];
+    int32_t reserved1;
+    double diffusivity[3];
+    /* ConstantSource (sources.jl:13-35): summed strength per field, 0 if none */
+    double source[3];
+    /* Pressure (pressure.jl:215-305) */
+    int32_t pressure;
+    /* MomentumAdvection (momentum_advection.jl:31-77): physical grid size */
+    int32_t advection;
+    int32_t adv_n1, adv_n2;
+    /* StaticSmagorinskyModel (static_smagorinsky.jl:44-123) */
+    int32_t sgs;
+    int32_t sgs_n1, sgs_n2;
+    int32_t sgs_wall[2];    /* bld_wall_kind, lower and upper */
+    int32_t reserved2;
+    const double *sgs_lc;   /* [n3]   length scale with wall damping, C-nodes */
+    const double *sgs_li;   /* [n3-1] same at interior I-nodes */
+    double sgs_factor_strain[2];  /* factors_strain (:92-108) */
+    double sgs_factor_stress[2];  /* factors_stress */
+} bld_config;
+
+typedef struct bld_engine bld_engine;
+typedef struct bld_ldlt bld_ldlt;
+
+/* Model construction: allocates device state, rates, SSPRK33 buffer, FFT plans
+ * and the pressure factorisation (src/BoundaryLayerDynamics.jl:79-96). */
+int bld_create(const bld_config *cfg, bld_engine **out);
+void bld_destroy(bld_engine *e);
+
+/* Message of the last failed call on `e` (or of the last failed bld_create
+ * on this thread when e is NULL). */
+const char *bld_last_error(const bld_engine *e);
+
+/* State transfer in the layout above (model.state, State.jl:9). */
+int bld_set_state(bld_engine *e, const double *vel1, const double *vel2, const double *vel3);
+int bld_get_state(bld_engine *e, double *vel1, double *vel2, double *vel3);
+
+/* compute_rates! (src/Processes.jl:39-62) of the current state, plus the
+ * divergence check of src/BoundaryLayerDynamics.jl:155-157.  Rates are written
+ * to the three host arrays (same layout as the state); pass NULL to skip. */
+int bld_rates(bld_engine *e, double *du1, double *du2, double *du3);
+
+/* apply_projections! (src/Processes.jl:67-75) on the current state. */
+int bld_project(bld_engine *e);
+
+/* ODEProblem setup (src/ODEMethods.jl:179-185): project the state and
+ * evaluate the rate at t1. */
+int bld_init_problem(bld_engine *e, double t1);
+
+/* nsteps × (perform_step!(SSPRK33) + rate!) (src/ODEMethods.jl:147-166,
+ * 226-231).  *t is advanced with the reference's snapping rule when t_end is
+ * given (t_end <= *t disables snapping).  Returns BLD_EDIVERGED if any rate
+ * became non-finite. */
+int bld_step(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end);
+
+/* Same as bld_step but records the whole step sequence into a HIP graph the
+ * first time and replays it afterwards (no host work per step). */
+int bld_step_graph(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end);
+
+/* set_field!/get_field! at the default physical size (2+2k1max, 2+2k2max)
+ * (src/physical_space.jl:253-312): `values` is real float64 [n1][n2][layers]
+ * column-major (x1 fastest).  field: 0 vel1, 1 vel2, 2 vel3. */
+int bld_set_field_physical(bld_engine *e, int field, const double *values);
+int bld_get_field_physical(bld_engine *e, int field, double *values);
+
+/* Engine timing hook: runs nsteps steps with inputs resident, returns the
+ * elapsed milliseconds measured with HIP events on the engine stream. */
+int bld_time_steps(bld_engine *e, double dt, int64_t nsteps, int use_graph, double *ms);
+
+/* Per-kernel timing: average duration (ms) of each kernel class over the last
+ * bld_time_steps call when profiling is enabled (bld_set_profiling). */
+int bld_set_profiling(bld_engine *e, int enabled);
+int bld_kernel_times(bld_engine *e, int kind, double *ms_total, int64_t *launches);
+
+/* Batched symmetric tridiagonal LDLt (DistributedBatchLDLt + ldiv!,
+ * src/processes/pressure.jl:38-201): nk systems of size nz.  dvs [nz][nk],
+ * evs [nz-1][nk] (mode index fastest).  precision: BLD_FP64 solves in float64;
+ * BLD_FP32_MIXED stores the right-hand side in float32, eliminates in float64
+ * and applies one step of iterative refinement with a float64 residual. */
+int bld_ldlt_create(int64_t nk, int64_t nz, const double *dvs, const double *evs, int precision,
+                    bld_ldlt **out);
+void bld_ldlt_destroy(bld_ldlt *s);
+/* In-place solve of complex float64 b [nz][nk] (interleaved), host memory. */
+int bld_ldlt_solve(bld_ldlt *s, double *b);
+/* Device-resident solve: uploads b once, then times `reps` solves (ms each). */
+int bld_ldlt_bench(bld_ldlt *s, const double *b, int reps, double *ms_per_solve, double *x_out);
+const char *bld_ldlt_last_error(const bld_ldlt *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLD_H */

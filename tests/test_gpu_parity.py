@@ -1,0 +1,264 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Float64 plain form: rates, projections and multi-step runs must agree with the
+oracle to 1e-10 relative; single right-hand sides agree
+to ~1e-13.  Float32 mixed form: 2e-5 relative for one right-hand side.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load_package
+
+pytestmark = pytest.mark.gpu
+
+B = load_package()
+
+RTOL64 = 1e-10  # float64 tolerance of the step engine against the oracle
+RTOL_RHS64 = 1e-12  # one right-hand side / projection
+RTOL_RHS32 = 2e-5  # float32 mixed form, one right-hand side
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def random_state(model, rng, amp=0.1, decay=0.5, hermitian=True):
+    K1, K2 = model.grid.fdsize()
+    k1, k2 = O.wavenumbers(model.grid)
+    scale = np.exp(-decay * np.sqrt(k1[:, None] ** 2 + k2[None, :] ** 2))[:, :, None]
+    out = {}
+    for f in model.fields:
+        n = model.state[f].shape[2]
+        a = (rng.standard_normal((K1, K2, n)) + 1j * rng.standard_normal((K1, K2, n))) * amp * scale
+        if hermitian:  # real field: k1 = 0 column Hermitian in k2
+            km = model.grid.k2max
+            a[0, K2 - km:, :] = np.conj(a[0, 1:km + 1, :][:, ::-1, :])
+            a[0, 0, :] = a[0, 0, :].real
+        out[f] = np.asfortranarray(a)
+    return out
+
+
+CASES = {
+    # closed channel DNS, smooth walls (C1-like, small)
+    "channel": dict(res=(12, 10, 8), dims=(2 * np.pi, np.pi, 2.0), lower="SmoothWall", upper="SmoothWall",
+                    procs=lambda M: M.incompressible_flow(1 / 180, constant_forcing=1)),
+    # neutral half channel LES, rough wall + free slip (C2-like, small)
+    "les": dict(res=(16, 16, 8), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4), upper="FreeSlipBoundary",
+                procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
+                                                      sgs_model=M.StaticSmagorinskyModel(Cs=0.1, wall_damping=True))),
+    # closed channel LES with two rough walls, no damping (les_test.jl:1-48)
+    "les_closed": dict(res=(12, 14, 6), dims=(2 * np.pi, 4 / 3 * np.pi, 2.0), lower=("RoughWall", 5e-3),
+                       upper=("RoughWall", 5e-3),
+                       procs=lambda M: [M.StaticSmagorinskyModel(wall_damping=False)]),
+    # SGS with quadratic dealiasing (both spaces at the same size)
+    "les_dealiased": dict(res=(10, 12, 5), dims=(2.0, 3.0, 1.0), lower=("RoughWall", 1e-3), upper="FreeSlipBoundary",
+                          procs=lambda M: M.incompressible_flow(1e-3, constant_forcing=(1, 0.5),
+                                                                sgs_model=M.StaticSmagorinskyModel(dealiasing="quadratic"))),
+    # stretched grid, free slip both sides
+    "stretched": dict(res=(8, 12, 10), dims=(1.0, 2.0, 1.0), lower="SmoothWall", upper="SmoothWall",
+                      mapping=0.7, procs=lambda M: M.incompressible_flow(0.05, constant_forcing=2)),
+    # odd physical sizes (k = 1 → quadratic size 6; k = 2 → 8) and a single resolved mode
+    "tiny": dict(res=(3, 5, 4), dims=(1.0, 1.0, 1.0), lower="SmoothWall", upper="FreeSlipBoundary",
+                 procs=lambda M: M.incompressible_flow(0.1)),
+    # custom boundaries: Neumann/Dirichlet values in diffusion, no advection
+    "custom": dict(res=(12, 14, 7), dims=(2 * np.pi, 2 * np.pi, 1.0),
+                   lower=("Custom", dict(vel1=("neumann", 1.3), vel2=("dirichlet", 0.4), vel3=("dirichlet", 0.2))),
+                   upper=("Custom", dict(vel1=("neumann", -0.7), vel2=("dirichlet", 0.9), vel3=("dirichlet", 0.2))),
+                   procs=lambda M: [M.MolecularDiffusion(v, 0.3) for v in ("vel1", "vel2", "vel3")]
+                   + [M.Pressure(), M.ConstantSource("vel2", 0.25)]),
+}
+
+
+def _boundary(M, spec):
+    if isinstance(spec, str):
+        return getattr(M, spec)()
+    if spec[0] == "Custom":
+        return M.CustomBoundary(**spec[1])
+    return getattr(M, spec[0])(spec[1])
+
+
+def make(M, name, **kw):
+    c = CASES[name]
+    mapping = M.SinusoidalMapping(c["mapping"]) if "mapping" in c else None
+    dom = M.Domain(c["dims"], _boundary(M, c["lower"]), _boundary(M, c["upper"]), mapping)
+    return M.Model(c["res"], dom, c["procs"](M), **kw)
+
+
+def pair(name, precision="fp64", seed=1, **kw):
+    ref = make(O, name)
+    gpu = make(B, name, precision=precision)
+    st = random_state(ref, np.random.default_rng(seed), **kw)
+    for f in ref.fields:
+        ref.state[f][...] = st[f]
+    gpu.set_state(**st)
+    return ref, gpu
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_state_roundtrip(name):
+    ref, gpu = pair(name)
+    got = gpu.state
+    for f in ref.fields:
+        assert np.array_equal(got[f], ref.state[f])
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_rates_fp64(name):
+    ref, gpu = pair(name, seed=3)
+    rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
+    O.compute_rates(ref, rates, ref.state)
+    got = gpu.rates()
+    for f in ref.fields:
+        assert relerr(got[f], rates[f]) < RTOL_RHS64, (name, f, relerr(got[f], rates[f]))
+
+
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched"])
+def test_rates_nonhermitian_input(name):
+    """c2r semantics for a k1=0 column that is not Hermitian (FFTW/pocketfft
+    take the real part, physical_space.jl:293)."""
+    ref, gpu = pair(name, seed=4, hermitian=False)
+    rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
+    O.compute_rates(ref, rates, ref.state)
+    got = gpu.rates()
+    for f in ref.fields:
+        assert relerr(got[f], rates[f]) < RTOL_RHS64, (name, f)
+
+
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "tiny"])
+def test_projection_fp64(name):
+    ref, gpu = pair(name, seed=5)
+    O.apply_projections(ref, ref.state)
+    gpu.project()
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < RTOL_RHS64, (name, f)
+
+
+@pytest.mark.parametrize("name,steps,dt", [("channel", 5, 1e-3), ("les", 5, 5e-4), ("stretched", 4, 1e-3),
+                                           ("les_dealiased", 3, 1e-3), ("tiny", 6, 1e-2), ("custom", 4, 1e-3)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_evolve_fp64(name, steps, dt, graph):
+    ref, gpu = pair(name, seed=6)
+    O.evolve(ref, steps * dt, dt)
+    B.evolve(gpu, steps * dt, dt, graph=graph)
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < RTOL64, (name, f, relerr(got[f], ref.state[f]))
+
+
+@pytest.mark.parametrize("name", ["channel", "les", "tiny"])
+def test_physical_transfer(name):
+    ref, gpu = pair(name, seed=7)
+    n1, n2 = O.pdsize(ref.grid)
+    for f in ref.fields:
+        got = gpu[f]
+        exp = O.get_field(ref.state[f], n1, n2)
+        assert relerr(got, exp) < 1e-13
+    rng = np.random.default_rng(8)
+    for f in ref.fields:
+        vals = rng.standard_normal((n1, n2, ref.state[f].shape[2]))
+        gpu.set_physical(f, vals)
+        exp = np.zeros_like(ref.state[f])
+        O.add_field(exp, vals)
+        assert relerr(gpu.state[f], exp) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["channel", "les", "custom"])
+def test_rates_fp32_mixed(name):
+    ref, gpu = pair(name, precision="fp32", seed=9)
+    rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
+    O.compute_rates(ref, rates, ref.state)
+    got = gpu.rates()
+    for f in ref.fields:
+        assert relerr(got[f], rates[f]) < RTOL_RHS32, (name, f, relerr(got[f], rates[f]))
+
+
+def test_evolve_fp32_mixed_mean_mode():
+    """The (0,0) mode is advanced in float64 in the mixed form: the mean
+    profile of a forced channel keeps float64 accuracy where the fluctuations
+    carry float32 rounding."""
+    ref, gpu = pair("channel", precision="fp32", seed=10, amp=0.02)
+    ref.state["vel1"][0, 0, :] += 1.5
+    st = gpu.state
+    st["vel1"][0, 0, :] += 1.5
+    gpu.set_state(**st)
+    O.evolve(ref, 10 * 1e-3, 1e-3)
+    B.evolve(gpu, 10 * 1e-3, 1e-3)
+    got = gpu.state
+    mean_err = relerr(got["vel1"][0, 0, :], ref.state["vel1"][0, 0, :])
+    assert mean_err < 1e-9, mean_err
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < 1e-4
+
+
+def test_divergence_detected():
+    ref, gpu = pair("channel", seed=11)
+    st = gpu.state
+    st["vel1"][1, 1, 2] = np.nan
+    gpu.set_state(**st)
+    with pytest.raises(B._lib.BldError, match="diverged"):
+        gpu.rates()
+
+
+def test_unsupported_configs():
+    lbc = B.CustomBoundary(vel1="dirichlet", vel2="dirichlet", vel3=("dirichlet", 0.5))
+    with pytest.raises(B._lib.BldError, match="non-zero w"):
+        B.Model((4, 4, 4), B.Domain((1, 1, 1), lbc, lbc), [B.MomentumAdvection()])
+    with pytest.raises(B._lib.BldError, match="mean flow must be equal"):
+        B.Model((4, 4, 4), B.Domain((1, 1, 1), lbc, B.SmoothWall()), [B.Pressure()])
+    nbc = B.CustomBoundary(vel1="dirichlet", vel2="dirichlet", vel3="neumann")
+    with pytest.raises(B._lib.BldError, match="only supports Dirichlet"):
+        B.Model((4, 4, 4), B.Domain((1, 1, 1), nbc, nbc), [B.Pressure()])
+
+
+# -- batched LDLt (pressure.jl:38-201, pressure_test.jl:1-38) --------------------
+@pytest.mark.parametrize("N", [16, 3, 64])
+def test_ldlt_dense(N):
+    rng = np.random.default_rng(N)
+    nk = 6
+    dv, ev = rng.random(N) + 2.5, rng.random(N - 1)
+    scale = np.arange(1, nk + 1)[:, None]
+    b = rng.random((nk, N)) + 1j * rng.random((nk, N))
+    A = B.BatchLDLt(scale * dv, scale * ev)
+    x = A.solve(b)
+    T = np.diag(dv) + np.diag(ev, 1) + np.diag(ev, -1)
+    for ik in range(nk):
+        assert relerr((ik + 1) * T @ x[ik], b[ik]) < 1e-13
+    xo = O.BatchLDLt(scale * dv, scale * ev).solve(b.copy())
+    assert relerr(x, xo) < 1e-14
+
+
+def pressure_operator(nk1, nk2, nz, L=(2 * np.pi, 2 * np.pi), L3=1.0):
+    """The pressure matrix of pressure.jl:231-296 for a uniform grid."""
+    K1, K2 = nk1, nk2
+    k1 = np.arange(K1)
+    k2 = np.concatenate([np.arange((K2 + 1) // 2), np.arange(-(K2 // 2), 0)])
+    a = nz / L3
+    d3g3 = np.full(nz, -2 * a)
+    d3g3[0] = d3g3[-1] = -a
+    kk1, kk2 = np.tile(k1, K2), np.repeat(k2, K1)
+    shift = 4 * np.pi ** 2 * (kk1 ** 2 / L[0] ** 2 + kk2 ** 2 / L[1] ** 2)
+    dvs = d3g3[None, :] - shift[:, None] / a
+    dvs[0] = d3g3
+    dvs[0, -1] = -3 * a
+    evs = np.full((K1 * K2, nz - 1), a)
+    return dvs, evs
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_ldlt_pressure_operator(precision, tol):
+    """Pressure systems (C3 shape at reduced size), float64 and float32 + one
+    refinement step, against the oracle's float64 solution."""
+    dvs, evs = pressure_operator(32, 32, 64)
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(dvs.shape) + 1j * rng.standard_normal(dvs.shape)
+    b[0, -1] = 0
+    x = B.BatchLDLt(dvs, evs, precision=precision).solve(b)
+    xo = O.BatchLDLt(dvs, evs).solve(b.copy())
+    err = np.abs(x - xo).max() / np.abs(xo).max()
+    assert err < tol, err

@@ -1,0 +1,80 @@
+"""CPU-side checks of the C ABI library and the host discretisation (no GPU calls)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import REPO, load_package
+
+B = load_package()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = B._lib.load()
+    header = open(os.path.join(REPO, "include", "bld.h")).read()
+    declared = set(re.findall(r"\b(bld_\w+)\s*\(", header))
+    assert declared, "no declarations found"
+    assert declared == set(B._lib.EXPORTS)
+    for name in declared:
+        assert getattr(lib, name) is not None
+
+
+def test_config_struct_matches_header_layout():
+    # 4 ints, 2 ints, 2 doubles, 2 pointers, 6 bcs (24 B), ... computed by the C compiler rule
+    c = B._lib.BldConfig
+    assert c.bc.offset % 8 == 0 and c.sgs_lc.offset % 8 == 0
+    assert ctypes.sizeof(B._lib.BldBC) == 24
+
+
+CFGS = [
+    ((128, 128, 64), (2 * np.pi, 2 * np.pi, 1.0), ("RoughWall", 1e-4), "FreeSlipBoundary", None),
+    ((64, 64, 64), (2 * np.pi, np.pi, 2.0), "FreeSlipBoundary", "FreeSlipBoundary", None),
+    ((16, 12, 9), (1.0, 2.0, 1.0), ("RoughWall", 1e-3), ("RoughWall", 1e-3), 0.6),
+]
+
+
+@pytest.mark.parametrize("res,dims,lo,hi,eta", CFGS)
+def test_discretization_constants_match_oracle(res, dims, lo, hi, eta):
+    def mk(M):
+        b = lambda s: getattr(M, s)() if isinstance(s, str) else getattr(M, s[0])(s[1])
+        mapping = None if eta is None else M.SinusoidalMapping(eta)
+        return M.Domain(dims, b(lo), b(hi), mapping)
+
+    procs = lambda M: M.incompressible_flow(1e-4, sgs_model=M.StaticSmagorinskyModel())
+    d = B.model.Discretization(res, mk(B), procs(B))
+    ref = O.Model(res, mk(O), procs(O))
+    c = d.config
+    n3 = res[2]
+    assert (c.k1max, c.k2max, c.n3) == (ref.grid.k1max, ref.grid.k2max, n3)
+    assert np.allclose(np.ctypeslib.as_array(c.alpha_c, (n3,)), ref.alpha_c, rtol=1e-15)
+    ai = np.ctypeslib.as_array(c.alpha_i, (n3 + 1,))
+    assert np.allclose(ai[1:-1], ref.alpha_i, rtol=1e-15)
+    sgs = [pd for k, pd in ref.disc if k == "sgs"][0]
+    assert (c.sgs_n1, c.sgs_n2) == sgs["dims"]
+    adv = [pd for k, pd in ref.disc if k == "adv"][0]
+    assert (c.adv_n1, c.adv_n2) == adv["dims"]
+    assert np.allclose(np.ctypeslib.as_array(c.sgs_lc, (n3,)), sgs["lc"], rtol=1e-14)
+    assert np.allclose(np.ctypeslib.as_array(c.sgs_li, (n3 - 1,)), sgs["li"], rtol=1e-14)
+    assert np.allclose(list(c.sgs_factor_strain), sgs["fstrain"], rtol=1e-14)
+    assert np.allclose(list(c.sgs_factor_stress), sgs["fstress"], rtol=1e-14)
+    assert list(c.sgs_wall) == [1 if w == "rough" else 0 for w in sgs["walls"]]
+
+
+def test_sgs_rejects_smooth_wall():
+    dom = B.Domain((1, 1, 1), B.SmoothWall(), B.SmoothWall())
+    with pytest.raises(ValueError, match="rough-wall and free-slip"):
+        B.model.Discretization((8, 8, 4), dom, [B.StaticSmagorinskyModel()])
+
+
+def test_unsupported_processes_rejected_on_host():
+    dom = B.Domain((1, 1, 1), B.SmoothWall(), B.SmoothWall())
+    with pytest.raises(NotImplementedError):
+        B.incompressible_flow(1.0, constant_flux=1.0)
+    with pytest.raises(NotImplementedError):
+        B.model.Discretization((4, 4, 4), dom, [B.MolecularDiffusion("theta", 1.0)])
+    with pytest.raises(ValueError):
+        B.model.Discretization((4, 4, 2), dom, [B.Pressure()])
