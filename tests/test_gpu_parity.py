@@ -37,7 +37,7 @@ def random_state(model, rng, amp=0.1, decay=0.5, hermitian=True):
         a = (rng.standard_normal((K1, K2, n)) + 1j * rng.standard_normal((K1, K2, n))) * amp * scale
         if hermitian:  # real field: k1 = 0 column Hermitian in k2
             km = model.grid.k2max
-            a[0, K2 - km:, :] = np.conj(a[0, 1:km + 1, :][:, ::-1, :])
+            a[0, K2 - km:, :] = np.conj(a[0, 1:km + 1, :][::-1, :])
             a[0, 0, :] = a[0, 0, :].real
         out[f] = np.asfortranarray(a)
     return out
