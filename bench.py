@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X SSPRK33 time-step engine.
+
+Default workload: the 128×128×64 neutral half-channel LES (rough wall z0=1e-4,
+free slip top, static Smagorinsky with wall damping, SSPRK33, dt = 5e-4).  The
+chosen form (float32 fields, float64 pressure elimination / (0,0) mode /
+forcing) is timed against the plain float64 form on one GPU.  A step is one
+SSPRK33 step (3 right-hand sides, 3 fused stage+projection passes), with the
+state resident in HBM; steps are replayed from a captured HIP graph.
+
+Prints ONE JSON line (see README of the metric in BASELINE.json).
+"""
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(REPO, "boundarylayerdynamics.jl_amd")
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+METRIC = "steps/s & grid-pt updates/s, 128²×64 LES, chosen form vs FP64 plain, 1 GPU"
+
+
+def load_package():
+    if "bld_amd" in sys.modules:
+        return sys.modules["bld_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "bld_amd", os.path.join(PKG_DIR, "__init__.py"), submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["bld_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_les(B, precision, res):
+    W = B.workloads
+    n, dom, procs = W.les_neutral(B, res)
+    m = B.Model(n, dom, procs, precision=precision)
+    n1, n2 = m.default_size()
+    u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2)
+    m.set_physical("vel1", u)
+    m.set_physical("vel2", v)
+    m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
+    m.init_problem(0.0)  # projection + first rate (ODEProblem)
+    return m
+
+
+def time_form(B, precision, res, dt, steps, warmup, profile_steps):
+    m = make_les(B, precision, res)
+    m.step(dt, warmup, graph=True)
+    t0 = time.perf_counter()
+    ms_events = m.time_steps(dt, steps, graph=True)
+    wall = time.perf_counter() - t0
+    kt = None
+    if profile_steps:
+        m.set_profiling(True)
+        m.step(dt, profile_steps, graph=False)
+        kt = m.kernel_times()
+        m.set_profiling(False)
+    st = m.state
+    finite = all(np.all(np.isfinite(a)) for a in st.values())
+    return m, dict(ms_per_step=1e3 * wall / steps, ms_per_step_events=ms_events / steps, finite=finite), kt
+
+
+def roofline(B, m, kt, precision, res, adv_n, sgs_n):
+    sizes = dict(K1=m.k1max + 1, K2=2 * m.k2max + 1, n3=m.n3,
+                 spaces=[("adv", *adv_n), ("sgs", *sgs_n)])
+    ab = B.workloads.algorithmic_bytes(sizes, None, precision)
+    per = {}
+    for k, (ms, n) in kt.items():
+        if n == 0 or k not in ab:
+            continue
+        avg_ms = ms / n
+        per[k] = dict(launches=n, avg_us=1e3 * avg_ms, total_ms=ms, bytes_per_launch=ab[k],
+                      gbs=ab[k] / (avg_ms * 1e-3) / 1e9)
+    dom = max(per, key=lambda k: per[k]["total_ms"])
+    d = per[dom]
+    return dict(kernel=dom, bound="hbm", achieved=round(d["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(d["gbs"] / HBM_PEAK_GBS, 4), traffic=None,
+                avg_us=round(d["avg_us"], 2), bytes_per_launch=int(d["bytes_per_launch"])), per
+
+
+def cpu_baseline(res, dt, budget_s):
+    """The oracle (float64 NumPy/scipy.fft restatement of the reference) on a
+    bounded sample of the same workload: a few SSPRK33 steps on the host cores."""
+    sys.path.insert(0, REPO)
+    import oracle as O
+    import oracle.model as OM
+    import scipy.fft  # noqa: F401
+
+    B = load_package()
+    W = B.workloads
+    n, dom, procs = W.les_neutral(O, res)
+    m = O.Model(n, dom, procs)
+    n1, n2 = O.pdsize(m.grid)
+    u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2)
+    for f in m.state.values():
+        f[...] = 0
+    O.add_field(m.state["vel1"], u)
+    O.add_field(m.state["vel2"], v)
+    cores = os.cpu_count() or 1
+    OM.FFT_WORKERS = cores
+    prob = O.ODEProblem(OM.rate_fn(m), [m.state[f] for f in m.fields], (0.0, 1e9),
+                        projection=OM.projection_fn(m))
+    buf = {"utmp": [np.zeros_like(x) for x in prob.u]}
+    from oracle.ode import perform_step
+    perform_step(prob, dt, O.SSPRK33(), buf)  # warm-up
+    prob.rate(prob.du, prob.u, dt)
+    nsteps, t0 = 0, time.perf_counter()
+    while True:
+        perform_step(prob, dt, O.SSPRK33(), buf)
+        prob.rate(prob.du, prob.u, dt)
+        nsteps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or nsteps >= 50:
+            break
+    return dict(value=round(nsteps / el, 4), unit="steps/s", cores=cores, kind="port",
+                sample=f"{nsteps} SSPRK33 steps (+1 warm-up) of the same {n[0]}x{n[1]}x{n[2]} LES in float64, "
+                       f"oracle (NumPy + scipy.fft with {cores} workers), {el:.1f} s")
+
+
+def pressure_solve(B, nz=256, n=256, reps=5):
+    """Isolated batched pressure solve (65,536 modes × nz), float64 and
+    float32 + one refinement step; bytes per solve = 32 B per unknown."""
+    K1, K2 = n, n
+    k1 = np.arange(K1)
+    k2 = np.concatenate([np.arange(K2 // 2), np.arange(-(K2 // 2), 0)])
+    a = float(nz)
+    d3g3 = np.full(nz, -2 * a)
+    d3g3[0] = d3g3[-1] = -a
+    kk1, kk2 = np.tile(k1, K2), np.repeat(k2, K1)
+    shift = 4 * np.pi ** 2 * (kk1 ** 2 + kk2 ** 2) / (2 * np.pi) ** 2
+    dvs = d3g3[None, :] - shift[:, None] / a
+    dvs[0] = d3g3
+    dvs[0, -1] = -3 * a
+    evs = np.full((K1 * K2, nz - 1), a)
+    rng = np.random.Generator(np.random.PCG64(3))
+    b = rng.standard_normal(dvs.shape) + 1j * rng.standard_normal(dvs.shape)
+    b[0, -1] = 0
+    out = {}
+    x64 = None
+    for prec in ("fp64", "fp32"):
+        s = B.BatchLDLt(dvs, evs, precision=prec)
+        ms, x = s.bench(b, reps=reps)
+        if prec == "fp64":
+            x64 = x
+            err = None
+        else:
+            err = float(np.abs(x - x64).max() / np.abs(x64).max())
+        byts = 32.0 * dvs.size
+        out[prec] = dict(ms_per_solve=round(ms, 4), gbs=round(byts / (ms * 1e-3) / 1e9, 1),
+                         rel_err_vs_fp64=err)
+    out["systems"] = int(K1 * K2)
+    out["nz"] = nz
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--dt", type=float, default=5e-4)
+    ap.add_argument("--res", type=str, default="128,128,64")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-pressure", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path "
+              f"in this build", file=sys.stderr)
+        sys.exit(2)
+    res = tuple(int(x) for x in args.res.split(","))
+    B = load_package()
+    log("fp32 mixed form ...")
+    m32, r32, kt32 = time_form(B, "fp32", res, args.dt, args.steps, args.warmup, args.profile_steps)
+    adv_n = B.pdsize((m32.k1max, m32.k2max), "quadratic")
+    sgs_n = B.pdsize((m32.k1max, m32.k2max), None)
+    roof, per32 = roofline(B, m32, kt32, "fp32", res, adv_n, sgs_n)
+    log("fp32:", r32, "dominant", roof["kernel"])
+    fp64 = None
+    if not args.no_fp64:
+        log("fp64 plain form ...")
+        m64, r64, kt64 = time_form(B, "fp64", res, args.dt, max(args.steps // 3, 10), max(args.warmup // 3, 3),
+                                   args.profile_steps)
+        roof64, per64 = roofline(B, m64, kt64, "fp64", res, adv_n, sgs_n)
+        fp64 = dict(steps_per_s=round(1e3 / r64["ms_per_step"], 2), ms_per_step=round(r64["ms_per_step"], 4),
+                    finite=r64["finite"], dominant_kernel=roof64["kernel"],
+                    kernels={k: dict(avg_us=round(v["avg_us"], 2), gbs=round(v["gbs"], 1)) for k, v in per64.items()})
+        del m64
+        log("fp64:", r64)
+    pres = None if args.no_pressure else pressure_solve(B)
+    cpu = None if args.no_cpu else cpu_baseline(res, args.dt, args.cpu_seconds)
+    sps = 1e3 / r32["ms_per_step"]
+    points = res[0] * res[1] * res[2]
+    line = {
+        "metric": METRIC,
+        "value": round(sps, 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 pressure elimination, (0,0) mode, forcing)",
+        "data": "synthetic: log-law profile + seeded Gaussian noise (sigma = 0.1 u*), projected",
+        "config": {"workload": f"neutral half-channel LES {res[0]}x{res[1]}x{res[2]}, rough wall z0=1e-4, "
+                               f"free-slip top, static Smagorinsky Cs=0.1 with wall damping, SSPRK33 dt={args.dt}",
+                   "form": "fp32 fields + fp64 pressure/mean mode (chosen form)",
+                   "advection_grid": list(adv_n), "sgs_grid": list(sgs_n), "hip_graph": True,
+                   "parallelism": "single GPU"},
+        "grid_point_updates_per_s": round(sps * points, 1),
+        "ms_per_step_hip_events": round(r32["ms_per_step_events"], 4),
+        "finite": r32["finite"],
+        "fp64_plain": fp64,
+        "speedup_vs_fp64_plain": round(sps / fp64["steps_per_s"], 3) if fp64 else None,
+        "kernels_fp32": {k: dict(avg_us=round(v["avg_us"], 2), launches=v["launches"], gbs=round(v["gbs"], 1))
+                         for k, v in per32.items()},
+        "roofline": roof,
+        "pressure_solve_65536x256": pres,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

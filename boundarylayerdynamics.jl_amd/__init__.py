@@ -12,3 +12,4 @@ from .model import (  # noqa: F401
 )
 from . import _lib  # noqa: F401
 from .ldlt import BatchLDLt  # noqa: F401
+from . import workloads  # noqa: F401

@@ -1,0 +1,89 @@
+"""Benchmark and parity workloads (the reference's channel/LES configurations).
+
+Each workload is a domain + process list + seeded initial condition, built
+against a module that provides the reference's model API (this package, or
+any object with the same names), so the same specification can be run by the
+GPU engine and by a CPU implementation.
+"""
+
+import math
+
+import numpy as np
+
+
+def les_neutral(M, n=(128, 128, 64), Cs=0.1):
+    """Neutral atmospheric half channel LES: 2π×2π×1, rough wall z0=1e-4, κ=0.4,
+    free-slip top, static Smagorinsky with Mason wall damping, ν=1e-5,
+    constant mean pressure gradient forcing 1 (les_test.jl:55-58 style)."""
+    dom = M.Domain((2 * math.pi, 2 * math.pi, 1.0), M.RoughWall(1e-4, 0.4), M.FreeSlipBoundary())
+    procs = M.incompressible_flow(1e-5, constant_forcing=1,
+                                  sgs_model=M.StaticSmagorinskyModel(Cs=Cs, wall_damping=True))
+    return n, dom, procs
+
+
+def les_initial(domain_z, n1, n2, seed=7, z0=1e-4, kappa=0.4, sigma=0.1):
+    """Log-law profile u = ln(z/z0)/κ (u* = 1) + seeded Gaussian noise σ on
+    vel1 and vel2; vel3 = 0.  Arrays at the default physical size."""
+    zc = np.asarray(domain_z)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (n1, n2, len(zc))
+    u = np.log(zc / z0)[None, None, :] / kappa + sigma * rng.standard_normal(shape)
+    v = sigma * rng.standard_normal(shape)
+    return np.asfortranarray(u), np.asfortranarray(v)
+
+
+def dns_channel(M, n=(64, 64, 64)):
+    """Closed channel DNS: 2π×π×2, smooth walls, ν = 1/180, constant forcing 1."""
+    dom = M.Domain((2 * math.pi, math.pi, 2.0), M.SmoothWall(), M.SmoothWall())
+    return n, dom, M.incompressible_flow(1 / 180, constant_forcing=1)
+
+
+def dns_initial(zc, zi, n1, n2, seed=11, uc=1.5, amp=0.05):
+    """Parabola u = uc·(1-(z-1)²) (docs/src/workflow.md:101) + seeded uniform
+    noise of amplitude amp·uc on all components."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    zc, zi = np.asarray(zc), np.asarray(zi)
+    noise = lambda nz: amp * uc * (2 * rng.random((n1, n2, nz)) - 1)
+    u = uc * (1 - (zc - 1) ** 2)[None, None, :] + noise(len(zc))
+    v = noise(len(zc))
+    w = noise(len(zi))
+    return np.asfortranarray(u), np.asfortranarray(v), np.asfortranarray(w)
+
+
+def algorithmic_bytes(cfg_sizes, kind, precision):
+    """Minimal HBM bytes of one launch of each kernel class (inputs read once,
+    outputs written once).  ``cfg_sizes`` holds K1, K2, n3 and the physical
+    sizes of the advection and SGS spaces."""
+    bc = 8 if precision == "fp32" else 16
+    br = bc // 2
+    K1, K2, n3 = cfg_sizes["K1"], cfg_sizes["K2"], cfg_sizes["n3"]
+    nk = K1 * K2
+    S = nk * (3 * n3 - 1) * bc  # spectral velocity state
+    spaces = cfg_sizes["spaces"]  # list of (kind, N1, N2)
+    nI, nC = n3 - 1, n3
+
+    def nk1(N1):
+        return min(N1 // 2 + 1, K1)
+
+    terms = {"adv": (3 * nC + 3 * nI), "sgs": (6 * nC + 3 * nI)}
+    srcs = {"adv": (2 * nC + nI), "sgs": (5 * nC + 3 * nI)}
+    out = {}
+    out["linear"] = 2 * S
+    out["c2r_x2"] = sum(terms[k] * (nk * bc + N2 * nk1(N1) * bc) for k, N1, N2 in spaces)
+    out["c2r_x1"] = sum(terms[k] * (N2 * nk1(N1) * bc + N1 * N2 * br) for k, N1, N2 in spaces)
+    out["r2c_x1"] = sum(srcs[k] * (N1 * N2 * br + N2 * nk1(N1) * bc) for k, N1, N2 in spaces)
+    r2c_x2 = []
+    for k, N1, N2 in spaces:
+        per_tgt = [(1, nC), (1, nC), (1, nI)] if k == "adv" else [(3, nC), (3, nC), (3, nI)]
+        r2c_x2.append(sum(ns * L * N2 * nk1(N1) * bc + 2 * L * nk * bc for ns, L in per_tgt))
+    out["r2c_x2"] = sum(r2c_x2) / max(1, len(r2c_x2))  # per launch (one launch per space)
+    for k, N1, N2 in spaces:
+        P = N1 * N2
+        if k == "adv":
+            out["adv_products"] = P * br * ((3 * nC + 3 * nI) + (2 * nC + nI))
+        else:
+            out["sgs_viscosity"] = P * br * ((6 * nC + 3 * nI) + (nC + nI))
+            out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (nC + nI) + (5 * nC + 3 * nI))
+    out["stage_project"] = (3 + 4 + 4) / 3 * S  # stage 1: u,du→utmp; stages 2,3: u,utmp,du→out
+    out["check_finite"] = S
+    return out
