@@ -163,6 +163,8 @@ class Engine : public EngineBase {
     DevBuf d_xfer_phys;
     LinDesc<T> lin{};
     ProjDesc<T> proj{};
+    ProjDesc<double> proj00{};  // float64 (0,0) column of the mixed form
+    DevBuf d_gamma0;
     // launch tables of the right-hand side
     struct Pass {
         LaunchTable lt{};
@@ -317,6 +319,9 @@ class Engine : public EngineBase {
             }
         }
         upload(d_gamma, gamma);
+        std::vector<double> gamma0(n3);
+        for (int i = 0; i < n3; ++i) gamma0[i] = gamma[size_t(i) * nk];
+        upload(d_gamma0, gamma0);
         d_pbuf.alloc(size_t(n3) * nk * sizeof(cplx<double>));
     }
 
@@ -372,6 +377,31 @@ class Engine : public EngineBase {
             D.du00[f] = mixed ? d_du00[f].as<cplx<double>>() : nullptr;
         }
         D.flag = d_flag.as<int>();
+        D.ext_stride = 0;
+        if (!mixed) return;
+        ProjDesc<double> &Z = proj00;  // one column (k1 = k2 = 0) over the float64 shadows
+        Z.K1 = Z.K2 = Z.nk = 1;
+        Z.n3 = n3;
+        Z.has_pressure = cfg.pressure;
+        Z.D1 = d_D1.as<double>();  // D1[0] = D2[0] = 0
+        Z.D2 = d_D2.as<double>();
+        Z.alpha_c = d_ac.as<double>();
+        Z.alpha_i = d_ai.as<double>();
+        Z.gamma = cfg.pressure ? d_gamma0.as<double>() : nullptr;
+        Z.wbc[0] = D.wbc[0];
+        Z.wbc[1] = D.wbc[1];
+        Z.pbuf = nullptr;
+        for (int f = 0; f < 3; ++f) {
+            Z.u[f] = d_u00[f].as<cplx<double>>();
+            Z.ut[f] = d_ut00[f].as<cplx<double>>();
+            Z.du[f] = d_du00[f].as<cplx<double>>();
+            Z.u00[f] = Z.ut00[f] = Z.du00[f] = nullptr;
+            Z.du_add[f] = reinterpret_cast<const cplx<float> *>(d_du[f].p);
+            Z.mirror_u[f] = reinterpret_cast<cplx<float> *>(d_u[f].p);
+            Z.mirror_ut[f] = reinterpret_cast<cplx<float> *>(d_ut[f].p);
+        }
+        Z.flag = d_flag.as<int>();
+        Z.ext_stride = nk;
     }
 
     int add_space(int kind, int N1, int N2) {
@@ -499,7 +529,9 @@ class Engine : public EngineBase {
         int b = int(limit / (size_t(bufs) * n * sizeof(cplx<T>)));
         b = std::max(1, std::min(b, 16));
         b = std::min(b, std::max(1, avail));
-        return b;
+        int p2 = 1;  // the kernels split indices with shifts: keep B a power of two
+        while (p2 * 2 <= b) p2 *= 2;
+        return p2;
     }
 
     void add_entry(Pass &ps, int kind, int sidx) {
@@ -607,7 +639,7 @@ class Engine : public EngineBase {
     }
 
     void stage(int s, double dt) {
-        timed(KK_STAGE, [&] { launch_stage_project<T>(proj, s, dt, st); });
+        timed(KK_STAGE, [&] { launch_stage_project<T>(proj, s, dt, st, mixed ? &proj00 : nullptr); });
     }
 
     void check_du() {

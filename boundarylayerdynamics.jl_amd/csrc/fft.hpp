@@ -118,4 +118,62 @@ __device__ cplx<T> *fft_block(cplx<T> *a, cplx<T> *b, const FftPlan &p, int batc
     return a;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time lengths: the radix sequence, spans and strides are constants, so
+// the per-butterfly index arithmetic reduces to multiplies and shifts and the
+// stage loop unrolls.  Same factorisation and arithmetic as fft_block.
+// ---------------------------------------------------------------------------
+struct Radices {
+    int n;
+    int r[kMaxStages];
+};
+
+constexpr Radices factor_ct(int N) {
+    Radices f{0, {}};
+    int n = N;
+    while (n % 4 == 0) {
+        f.r[f.n++] = 4;
+        n /= 4;
+    }
+    for (int p = 2; p <= 5; ++p) {
+        if (p == 4) continue;
+        while (n % p == 0) {
+            f.r[f.n++] = p;
+            n /= p;
+        }
+    }
+    for (int p = 7; n > 1; p += 2) {
+        while (n % p == 0) {
+            f.r[f.n++] = p;
+            n /= p;
+        }
+    }
+    return f;
+}
+
+template <typename T, bool INV, int N, int S, int NS>
+__device__ __forceinline__ void fft_stages_ct(cplx<T> *&a, cplx<T> *&b, int batch, const cplx<T> *__restrict__ tw) {
+    constexpr Radices F = factor_ct(N);
+    if constexpr (S < F.n) {
+        constexpr int R = F.r[S];
+        constexpr int M = N / R;
+        for (int j = threadIdx.x; j < batch * M; j += blockDim.x) {
+            const int t = j / M;
+            const int jj = j - t * M;
+            stage_radix<T, INV>(a + t * N, b + t * N, N, R, NS, jj, tw);
+        }
+        __syncthreads();
+        cplx<T> *tmp = a;
+        a = b;
+        b = tmp;
+        fft_stages_ct<T, INV, N, S + 1, NS * R>(a, b, batch, tw);
+    }
+}
+
+template <typename T, bool INV, int N>
+__device__ __forceinline__ cplx<T> *fft_block_ct(cplx<T> *a, cplx<T> *b, int batch, const cplx<T> *__restrict__ tw) {
+    fft_stages_ct<T, INV, N, 0, 1>(a, b, batch, tw);
+    return a;
+}
+
 }  // namespace bld

@@ -69,15 +69,45 @@ __device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int r
     }
 }
 
+// Division by a block-uniform runtime divisor d (n·d < 2^32): q = ⌊n·m/2^32⌋
+// with m = ⌈2^32/d⌉.
+struct Div {
+    uint32_t d, m;
+    __device__ explicit Div(int dd) : d(uint32_t(dd)), m(uint32_t(0xFFFFFFFFu / uint32_t(dd)) + 1u) {}
+    __device__ __forceinline__ int q(int n) const { return d == 1 ? n : int(__umulhi(uint32_t(n), m)); }
+};
+
+template <typename T, bool INV, int FN>
+__device__ __forceinline__ cplx<T> *run_fft(cplx<T> *a, cplx<T> *b, const FftPlan &p, int batch,
+                                           const cplx<T> *__restrict__ tw) {
+    if constexpr (FN > 0)
+        return fft_block_ct<T, INV, FN>(a, b, batch, tw);
+    else
+        return fft_block<T, INV>(a, b, p, batch, tw);
+}
+
+// Lengths with compile-time FFT bodies; anything else takes the generic path.
+#define BLD_FFT_SWITCH(N, CALL)      \
+    switch (N) {                     \
+        case 16: CALL(16); break;    \
+        case 24: CALL(24); break;    \
+        case 32: CALL(32); break;    \
+        case 48: CALL(48); break;    \
+        case 64: CALL(64); break;    \
+        case 96: CALL(96); break;    \
+        case 128: CALL(128); break;  \
+        case 144: CALL(144); break;  \
+        case 192: CALL(192); break;  \
+        case 256: CALL(256); break;  \
+        case 384: CALL(384); break;  \
+        default: CALL(0); break;     \
+    }
+
 // ---------------------------------------------------------------------------
 // c2r, pass 1: inverse FFT along x2 of `batch` k1-columns
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
-    extern __shared__ unsigned char smem_raw[];
-    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
-    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
-    const Space<T> &S = spaces[E.space];
+template <typename T, int FN>
+__device__ __forceinline__ void c2r_x2_body(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
     const int local = blockIdx.x - E.block_begin;
     const int job = local / E.blocks_per_job;
     const int chunk = local - job * E.blocks_per_job;
@@ -85,12 +115,14 @@ __global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spa
     while (job >= S.in_job0[term + 1]) ++term;
     const int layer = job - S.in_job0[term];
     const int recipe = S.in_recipe[term];
-    const int B = E.batch, N2 = S.N2;
+    const int B = E.batch;
+    const int N2 = FN > 0 ? FN : S.N2;
+    const Div divN(N2);
     cplx<T> *bufA = smem, *bufB = smem + B * N2;
     const int k1_0 = chunk * B;
     const int ncols = min(B, S.nk1 - k1_0);
     for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
-        const int c = idx / N2, pos = idx - c * N2;
+        const int c = divN.q(idx), pos = idx - c * N2;
         cplx<T> val = czero<T>();
         if (c < ncols) {
             int i2 = -1;
@@ -103,39 +135,50 @@ __global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spa
         bufA[idx] = val;
     }
     __syncthreads();
-    cplx<T> *res = fft_block<T, true>(bufA, bufB, S.p2, B, S.tw + S.p2.tw);
+    cplx<T> *res = run_fft<T, true, FN>(bufA, bufB, S.p2, B, S.tw + S.p2.tw);
     cplx<T> *out = S.inter + size_t(job) * N2 * S.nk1;
-    for (int idx = threadIdx.x; idx < ncols * N2; idx += blockDim.x) {
-        const int x2 = idx / ncols, c = idx - x2 * ncols;
+    const int lb = __ffs(B) - 1;  // B is a power of two
+    for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
+        const int x2 = idx >> lb, c = idx & (B - 1);
+        if (c >= ncols) continue;
         cplx<T> v = res[c * N2 + x2];
         if (k1_0 + c == 0) v.y = T(0);  // c2r uses the real part of the k1 = 0 column
         out[size_t(x2) * S.nk1 + k1_0 + c] = v;
     }
 }
 
-// ---------------------------------------------------------------------------
-// c2r, pass 2: Hermitian completion along x1, two real rows per complex FFT
-// ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+__global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n) c2r_x2_body<T, n>(S, E, sel, smem)
+    BLD_FFT_SWITCH(S.N2, BLD_CALL)
+#undef BLD_CALL
+}
+
+// ---------------------------------------------------------------------------
+// c2r, pass 2: Hermitian completion along x1, two real rows per complex FFT
+// ---------------------------------------------------------------------------
+template <typename T, int FN>
+__device__ __forceinline__ void c2r_x1_body(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
     const int local = blockIdx.x - E.block_begin;
     const int job = local / E.blocks_per_job;
     const int chunk = local - job * E.blocks_per_job;
     int term = 0;
     while (job >= S.in_job0[term + 1]) ++term;
     const int layer = job - S.in_job0[term];
-    const int B = E.batch, N1 = S.N1, N2 = S.N2, k1m = S.k1m, nk1 = S.nk1;
+    const int B = E.batch, N2 = S.N2, k1m = S.k1m, nk1 = S.nk1;
+    const int N1 = FN > 0 ? FN : S.N1;
+    const Div divN(N1);
     const int npairs_tot = (N2 + 1) / 2;
     const int pair0 = chunk * B;
     const int np = min(B, npairs_tot - pair0);
     cplx<T> *bufA = smem, *bufB = smem + B * N1;
     const cplx<T> *in = S.inter + size_t(job) * N2 * nk1;
     for (int idx = threadIdx.x; idx < B * N1; idx += blockDim.x) {
-        const int p = idx / N1, k = idx - p * N1;
+        const int p = divN.q(idx), k = idx - p * N1;
         cplx<T> z = czero<T>();
         if (p < np) {
             const int xa = 2 * (pair0 + p), xb = xa + 1;
@@ -160,10 +203,10 @@ __global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spa
         bufA[idx] = z;
     }
     __syncthreads();
-    cplx<T> *res = fft_block<T, true>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
+    cplx<T> *res = run_fft<T, true, FN>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
     T *out = S.in_phys[term] + size_t(layer) * N1 * N2;
     for (int idx = threadIdx.x; idx < np * N1; idx += blockDim.x) {
-        const int p = idx / N1, x = idx - p * N1;
+        const int p = divN.q(idx), x = idx - p * N1;
         const int xa = 2 * (pair0 + p);
         cplx<T> z = res[p * N1 + x];
         out[size_t(xa) * N1 + x] = z.x;
@@ -171,29 +214,38 @@ __global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spa
     }
 }
 
-// ---------------------------------------------------------------------------
-// r2c, pass 1: forward FFT along x1 of two real rows at once, truncated to k1m
-// ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+__global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n) c2r_x1_body<T, n>(S, E, smem)
+    BLD_FFT_SWITCH(S.N1, BLD_CALL)
+#undef BLD_CALL
+}
+
+// ---------------------------------------------------------------------------
+// r2c, pass 1: forward FFT along x1 of two real rows at once, truncated to k1m
+// ---------------------------------------------------------------------------
+template <typename T, int FN>
+__device__ __forceinline__ void r2c_x1_body(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
     const int local = blockIdx.x - E.block_begin;
     const int job = local / E.blocks_per_job;
     const int chunk = local - job * E.blocks_per_job;
     int src = 0;
     while (job >= S.src_job0[src + 1]) ++src;
     const int layer = job - S.src_job0[src];
-    const int B = E.batch, N1 = S.N1, N2 = S.N2, nk1 = S.nk1;
+    const int B = E.batch, N2 = S.N2, nk1 = S.nk1;
+    const int N1 = FN > 0 ? FN : S.N1;
+    const Div divN(N1), divK(nk1);
     const int npairs_tot = (N2 + 1) / 2;
     const int pair0 = chunk * B;
     const int np = min(B, npairs_tot - pair0);
     cplx<T> *bufA = smem, *bufB = smem + B * N1;
     const T *in = S.src_phys[src] + size_t(layer) * N1 * N2;
     for (int idx = threadIdx.x; idx < B * N1; idx += blockDim.x) {
-        const int p = idx / N1, x = idx - p * N1;
+        const int p = divN.q(idx), x = idx - p * N1;
         cplx<T> z = czero<T>();
         if (p < np) {
             const int xa = 2 * (pair0 + p);
@@ -203,10 +255,10 @@ __global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spa
         bufA[idx] = z;
     }
     __syncthreads();
-    cplx<T> *res = fft_block<T, false>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
+    cplx<T> *res = run_fft<T, false, FN>(bufA, bufB, S.p1, B, S.tw + S.p1.tw);
     cplx<T> *out = S.inter + size_t(job) * N2 * nk1;
     for (int idx = threadIdx.x; idx < np * nk1; idx += blockDim.x) {
-        const int p = idx / nk1, k = idx - p * nk1;
+        const int p = divK.q(idx), k = idx - p * nk1;
         const int xa = 2 * (pair0 + p);
         const cplx<T> zk = res[p * N1 + k];
         const cplx<T> zn = res[p * N1 + (k == 0 ? 0 : N1 - k)];
@@ -215,30 +267,39 @@ __global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spa
     }
 }
 
-// ---------------------------------------------------------------------------
-// r2c, pass 2: forward FFT along x2, truncation, 1/(N1·N2), prefactors,
-// accumulation into the spectral rate (one block owns each output element)
-// ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+__global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n) r2c_x1_body<T, n>(S, E, smem)
+    BLD_FFT_SWITCH(S.N1, BLD_CALL)
+#undef BLD_CALL
+}
+
+// ---------------------------------------------------------------------------
+// r2c, pass 2: forward FFT along x2, truncation, 1/(N1·N2), prefactors,
+// accumulation into the spectral rate (one block owns each output element)
+// ---------------------------------------------------------------------------
+template <typename T, int FN>
+__device__ __forceinline__ void r2c_x2_body(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
     const int local = blockIdx.x - E.block_begin;
     const int job = local / E.blocks_per_job;
     const int chunk = local - job * E.blocks_per_job;
     int tg = 0;
     while (job >= S.tgt_job0[tg + 1]) ++tg;
     const int layer = job - S.tgt_job0[tg];
-    const int B = E.batch, N2 = S.N2, nk1 = S.nk1;
+    const int B = E.batch, nk1 = S.nk1;
+    const int N2 = FN > 0 ? FN : S.N2;
     const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
     const int nT = nH > 0 ? 2 * B : B;  // G columns, then H columns
     cplx<T> *bufA = smem, *bufB = smem + nT * N2;
     const int k1_0 = chunk * B;
     const int ncols = min(B, nk1 - k1_0);
+    const int lb = __ffs(B) - 1;  // B is a power of two
     for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
-        const int x2 = idx / B, c = idx - x2 * B;  // c fastest: coalesced over k1
+        const int x2 = idx >> lb, c = idx & (B - 1);  // c fastest: coalesced over k1
         cplx<T> g = czero<T>(), h = czero<T>();
         if (c < ncols) {
             const int k1 = k1_0 + c;
@@ -257,12 +318,13 @@ __global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spa
         if (nH > 0) bufA[(B + c) * N2 + x2] = h;
     }
     __syncthreads();
-    cplx<T> *res = fft_block<T, false>(bufA, bufB, S.p2, nT, S.tw + S.p2.tw);
-    const T scale = T(1.0 / (double(S.N1) * double(S.N2)));
+    cplx<T> *res = run_fft<T, false, FN>(bufA, bufB, S.p2, nT, S.tw + S.p2.tw);
+    const T scale = T(1.0 / (double(S.N1) * double(N2)));
     const int nres = 2 * S.k2m + 1;
     cplx<T> *out = S.tgt_out[tg] + size_t(layer) * S.K1 * S.K2;
-    for (int idx = threadIdx.x; idx < ncols * nres; idx += blockDim.x) {
-        const int r = idx / ncols, c = idx - r * ncols;
+    for (int idx = threadIdx.x; idx < B * nres; idx += blockDim.x) {
+        const int r = idx >> lb, c = idx & (B - 1);
+        if (c >= ncols) continue;
         const int i2 = r <= S.k2m ? r : S.K2 - (nres - r);
         const int pos = r <= S.k2m ? r : N2 - (nres - r);
         cplx<T> v = res[c * N2 + pos];
@@ -270,6 +332,17 @@ __global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spa
         const size_t o = size_t(i2) * S.K1 + k1_0 + c;
         out[o] = out[o] + scale * v;
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n) r2c_x2_body<T, n>(S, E, smem)
+    BLD_FFT_SWITCH(S.N2, BLD_CALL)
+#undef BLD_CALL
 }
 
 // ---------------------------------------------------------------------------
@@ -559,9 +632,9 @@ __device__ __forceinline__ cplx<A> combine(int stage, cplx<A> u, cplx<A> ut, cpl
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, double dt) {
+__global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, double dt, int mode_limit) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= D.nk) return;
+    if (m >= D.nk || m >= mode_limit) return;
     const int nk = D.nk, n3 = D.n3;
     const bool mixed00 = (m == 0) && (D.u00[0] != nullptr);
     // input/output selection: stage 0 projects u in place, 1-2 write utmp, 3 writes u
@@ -680,6 +753,211 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
     if (bad) *D.flag = 1;
 }
 
+
+// Same stage + projection with each (k1,k2) column split into chunks of CL
+// levels held in registers by different threads: lanes are consecutive modes
+// (coalesced), threadIdx / MB is the chunk.  Both triangular sweeps of the
+// LDLt solve are affine recurrences y_l = a_l·y_{l-1} + b_l; each chunk runs
+// its sweep from a zero incoming value while tracking the product of the a_l,
+// and the true values follow from one combine over the chunks through LDS.
+// In the first (forward) / last (backward) chunk this is the sequential
+// recurrence of pressure.jl:126-160 operation for operation.
+template <typename T, int CL>
+__global__ void __launch_bounds__(256) k_stage_chunked(ProjDesc<T> D, int stage, double dt, int MB, int skip0) {
+    extern __shared__ unsigned char smem_raw[];
+    const int NC = blockDim.x / MB;
+    cplx<double> *s_y = reinterpret_cast<cplx<double> *>(smem_raw);  // [NC][MB]
+    double *s_p = reinterpret_cast<double *>(s_y + NC * MB);         // [NC][MB]
+    cplx<T> *s_w = reinterpret_cast<cplx<T> *>(s_p + NC * MB);      // [NC][MB]
+    const int tx = threadIdx.x % MB, ty = threadIdx.x / MB;
+    const int m = blockIdx.x * MB + tx;
+    const int nk = D.nk, n3 = D.n3;
+    const bool active = m < nk && !(skip0 && m == 0);
+    const int l0 = ty * CL;
+    const int mm = active ? m : 0;
+    cplx<T> **outp = (stage == 1 || stage == 2) ? D.ut : D.u;
+    bool bad = false;
+    cplx<T> c1[CL], c2[CL], c3[CL];
+#pragma unroll
+    for (int j = 0; j < CL; ++j) {
+        const int l = l0 + j;
+        c1[j] = c2[j] = c3[j] = mk(T(0), T(0));
+        if (active && l < n3) {
+            for (int f = 0; f < 3; ++f) {
+                if (f == 2 && l == n3 - 1) break;
+                const size_t i = size_t(l) * nk + m;
+                cplx<T> u = D.u[f][i], ut = mk(T(0), T(0)), du = mk(T(0), T(0));
+                if (stage == 2 || stage == 3) ut = D.ut[f][i];
+                if (stage > 0) {
+                    du = D.du[f][i];
+                    if (D.ext_stride) {
+                        const cplx<float> e = D.du_add[f][size_t(l) * D.ext_stride];
+                        du = mk(du.x + T(e.x), du.y + T(e.y));
+                    }
+                    if (!(isfinite(du.x) && isfinite(du.y))) bad = true;
+                }
+                const cplx<T> c = combine<T, T>(stage, u, ut, du, T(dt));
+                if (f == 0) c1[j] = c;
+                else if (f == 1) c2[j] = c;
+                else c3[j] = c;
+            }
+        }
+    }
+    if (bad) *D.flag = 1;
+    if (!D.has_pressure) {
+        if (stage > 0 && active) {
+#pragma unroll
+            for (int j = 0; j < CL; ++j) {
+                const int l = l0 + j;
+                if (l >= n3) break;
+                const size_t i = size_t(l) * nk + m;
+                outp[0][i] = c1[j];
+                outp[1][i] = c2[j];
+                if (l < n3 - 1) outp[2][i] = c3[j];
+                if (D.ext_stride) {
+                    cplx<float> **mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+                    const size_t e = size_t(l) * D.ext_stride;
+                    mir[0][e] = mk(float(c1[j].x), float(c1[j].y));
+                    mir[1][e] = mk(float(c2[j].x), float(c2[j].y));
+                    if (l < n3 - 1) mir[2][e] = mk(float(c3[j].x), float(c3[j].y));
+                }
+            }
+        }
+        return;
+    }
+    const int k1 = mm % D.K1, i2 = mm / D.K1;
+    const double d1 = D.D1[k1], d2 = D.D2[i2];
+    // w at the I-level below this chunk (last level of the previous chunk)
+    {
+        const int jl = min(CL, n3 - l0) - 1;
+        cplx<T> last = mk(T(0), T(0));
+#pragma unroll
+        for (int j = 0; j < CL; ++j)
+            if (j == jl) last = c3[j];
+        s_w[ty * MB + tx] = last;
+    }
+    __syncthreads();
+    cplx<double> wb = mk(0.0, 0.0);
+    if (ty > 0) {
+        const cplx<T> v = s_w[(ty - 1) * MB + tx];
+        wb = mk(double(v.x), double(v.y));
+    }
+    // forward sweep from a zero incoming value
+    cplx<double> y[CL];
+    double P[CL], g[CL];
+    cplx<double> yprev = mk(0.0, 0.0);
+    double Pprev = 1.0;
+#pragma unroll
+    for (int j = 0; j < CL; ++j) {
+        const int l = l0 + j;
+        y[j] = mk(0.0, 0.0);
+        P[j] = 0.0;
+        g[j] = 0.0;
+        if (l < n3) {
+            const cplx<double> a1 = mk(double(c1[j].x), double(c1[j].y));
+            const cplx<double> a2 = mk(double(c2[j].x), double(c2[j].y));
+            const cplx<double> a3 = mk(double(c3[j].x), double(c3[j].y));
+            const double hf = 1.0 / D.alpha_c[l];
+            cplx<double> r = mk((-d1 * a1.y) * hf + (-d2 * a2.y) * hf, (d1 * a1.x) * hf + (d2 * a2.x) * hf);
+            if (l > 0) r = mk(r.x - wb.x, r.y - wb.y);
+            if (l < n3 - 1) r = mk(r.x + a3.x, r.y + a3.y);
+            if (m == 0) {
+                if (l == 0) r.x -= D.wbc[0];
+                if (l == n3 - 1) r.x += D.wbc[1];
+                if (l == n3 - 1) r = mk(0.0, 0.0);
+            }
+            const double gl = active ? D.gamma[size_t(l) * nk + m] : 1.0;
+            g[j] = gl;
+            if (l == 0) {
+                y[j] = mk(gl * r.x, gl * r.y);
+                P[j] = 0.0;
+            } else {
+                const double b = D.alpha_i[l];
+                y[j] = mk(gl * (r.x - b * yprev.x), gl * (r.y - b * yprev.y));
+                P[j] = -(gl * b) * Pprev;
+            }
+            yprev = y[j];
+            Pprev = P[j];
+            wb = a3;
+        }
+    }
+    s_y[ty * MB + tx] = yprev;
+    s_p[ty * MB + tx] = Pprev;
+    __syncthreads();
+    cplx<double> yin = mk(0.0, 0.0);
+    for (int c = 0; c < ty; ++c) {
+        const cplx<double> v = s_y[c * MB + tx];
+        const double pp = s_p[c * MB + tx];
+        yin = mk(v.x + pp * yin.x, v.y + pp * yin.y);
+    }
+    if (ty > 0) {
+#pragma unroll
+        for (int j = 0; j < CL; ++j) y[j] = mk(y[j].x + P[j] * yin.x, y[j].y + P[j] * yin.y);
+    }
+    __syncthreads();
+    // backward sweep from a zero value above the chunk
+    cplx<double> pnext = mk(0.0, 0.0);
+    double Qnext = 1.0;
+#pragma unroll
+    for (int j = CL - 1; j >= 0; --j) {
+        const int l = l0 + j;
+        if (l < n3) {
+            if (l == n3 - 1) {
+                P[j] = 0.0;  // Q
+            } else {
+                const double gb = g[j] * D.alpha_i[l + 1];
+                y[j] = mk(y[j].x - gb * pnext.x, y[j].y - gb * pnext.y);
+                P[j] = -gb * Qnext;
+            }
+            pnext = y[j];
+            Qnext = P[j];
+        }
+    }
+    s_y[ty * MB + tx] = pnext;  // p̃ at l0
+    s_p[ty * MB + tx] = Qnext;
+    __syncthreads();
+    cplx<double> pin = mk(0.0, 0.0);  // p at the level just above this chunk
+    for (int c = NC - 1; c > ty; --c) {
+        const cplx<double> v = s_y[c * MB + tx];
+        const double q = s_p[c * MB + tx];
+        pin = mk(v.x + q * pin.x, v.y + q * pin.y);
+    }
+    if (ty < NC - 1) {
+#pragma unroll
+        for (int j = 0; j < CL; ++j) y[j] = mk(y[j].x + P[j] * pin.x, y[j].y + P[j] * pin.y);
+    }
+    if (!active) return;
+    // gradient (add_gradient!, pressure.jl:405-426) and store
+#pragma unroll
+    for (int j = 0; j < CL; ++j) {
+        const int l = l0 + j;
+        if (l >= n3) break;
+        const size_t i = size_t(l) * nk + m;
+        const cplx<double> p = y[j];
+        cplx<double> v1 = mk(double(c1[j].x), double(c1[j].y));
+        cplx<double> v2 = mk(double(c2[j].x), double(c2[j].y));
+        v1 = mk(v1.x + d1 * p.y, v1.y - d1 * p.x);
+        v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
+        outp[0][i] = mk(T(v1.x), T(v1.y));
+        outp[1][i] = mk(T(v2.x), T(v2.y));
+        cplx<float> **mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+        const size_t e = size_t(l) * D.ext_stride;
+        if (D.ext_stride) {
+            mir[0][e] = mk(float(v1.x), float(v1.y));
+            mir[1][e] = mk(float(v2.x), float(v2.y));
+        }
+        if (l < n3 - 1) {
+            const cplx<double> pn = (j + 1 < CL) ? y[j + 1] : pin;
+            const double a = D.alpha_i[l + 1];
+            const cplx<double> gr = mk((-a) * p.x + a * pn.x, (-a) * p.y + a * pn.y);
+            cplx<double> v3 = mk(double(c3[j].x), double(c3[j].y));
+            v3 = mk(v3.x - gr.x, v3.y - gr.y);
+            outp[2][i] = mk(T(v3.x), T(v3.y));
+            if (D.ext_stride) mir[2][e] = mk(float(v3.x), float(v3.y));
+        }
+    }
+}
+
 // finite check of the rates (BoundaryLayerDynamics.jl:155-157)
 template <typename T>
 __global__ void __launch_bounds__(256) k_check_finite(const cplx<T> *a, size_t n, int *flag) {
@@ -727,8 +1005,32 @@ void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st) {
 }
 
 template <typename T>
-void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st) {
-    k_stage_project<T><<<(D.nk + 63) / 64, 64, 0, st>>>(D, stage, dt);
+static bool launch_chunked(const ProjDesc<T> &D, int stage, double dt, int skip0, hipStream_t st) {
+    int CL = D.n3 <= 64 ? 4 : (D.n3 <= 128 ? 8 : 16);
+    const int NC = (D.n3 + CL - 1) / CL;
+    if (NC > 16) return false;
+    const int MB = D.nk == 1 ? 1 : std::max(4, 256 / NC);  // modes per block (block = MB * NC <= 256)
+    const int nb = (D.nk + MB - 1) / MB;
+    const size_t smem = size_t(NC) * MB * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<T>));
+    const int threads = MB * NC;
+    if (CL == 4)
+        k_stage_chunked<T, 4><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
+    else if (CL == 8)
+        k_stage_chunked<T, 8><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
+    else
+        k_stage_chunked<T, 16><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
+    return true;
+}
+
+template <typename T>
+void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st,
+                          const ProjDesc<double> *D00) {
+    const bool mixed = D00 != nullptr;
+    if (launch_chunked<T>(D, stage, dt, mixed ? 1 : 0, st)) {
+        if (mixed) launch_chunked<double>(*D00, stage, dt, 0, st);  // float64 (0,0) mode
+        return;
+    }
+    k_stage_project<T><<<(D.nk + 63) / 64, 64, 0, st>>>(D, stage, dt, D.nk);
 }
 
 template <typename T>
@@ -747,7 +1049,7 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
     template void launch_transform<T>(int, const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
     template void launch_physics<T>(int, const Space<T> *, size_t, hipStream_t);                                  \
     template void launch_linear<T>(const LinDesc<T> &, int, hipStream_t);                                         \
-    template void launch_stage_project<T>(const ProjDesc<T> &, int, double, hipStream_t);                         \
+    template void launch_stage_project<T>(const ProjDesc<T> &, int, double, hipStream_t, const ProjDesc<double> *); \
     template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \
     template void launch_fill<T>(void *, size_t, hipStream_t);
 

@@ -39,6 +39,11 @@ struct ProjDesc {
     cplx<T> *u[3], *ut[3], *du[3];
     cplx<double> *u00[3], *ut00[3], *du00[3];
     int *flag;
+    // float64 (0,0)-mode pass of the mixed form: the float32 nonlinear rate of
+    // the mode is added to du, and results are mirrored into the float32 state
+    int ext_stride;                     // 0: no extras
+    const cplx<float> *du_add[3];
+    cplx<float> *mirror_u[3], *mirror_ut[3];
 };
 
 template <typename T>
@@ -48,8 +53,9 @@ template <typename T>
 void launch_physics(int kind, const Space<T> *space_dev, size_t npoints, hipStream_t st);
 template <typename T>
 void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st);
+// D00: float64 pass over the (0,0) column of the mixed form (null in the plain form)
 template <typename T>
-void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st);
+void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st, const ProjDesc<double> *D00);
 template <typename T>
 void launch_check_finite(const void *a, size_t n, int *flag, hipStream_t st);
 template <typename T>

@@ -262,3 +262,41 @@ def test_ldlt_pressure_operator(precision, tol):
     xo = O.BatchLDLt(dvs, evs).solve(b.copy())
     err = np.abs(x - xo).max() / np.abs(xo).max()
     assert err < tol, err
+
+
+# -- committed golden vectors (tests/golden/make_golden.py) ------------------------
+@pytest.mark.parametrize("name", ["les_12x14x16", "channel_8x8x6", "pressure_4x4x16"])
+def test_golden_vectors(name):
+    import os
+    import sys
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    sys.path.insert(0, here)
+    import make_golden as G
+    g = np.load(os.path.join(here, f"{name}.npz"))
+    ref, dt = G.build(name)
+    res, dims, lo, hi, procs, _ = G.SPECS[name]
+    bnd = lambda s: B.RoughWall(s[1]) if s[0] == "rough" else (B.FreeSlipBoundary() if s[0] == "freeslip"
+                                                               else B.SmoothWall())
+    # the same processes through the host mirror
+    if name.startswith("les"):
+        pr = B.incompressible_flow(1e-4, constant_forcing=(1, 0.2), sgs_model=B.StaticSmagorinskyModel())
+    elif name.startswith("channel"):
+        pr = B.incompressible_flow(1 / 180, constant_forcing=1)
+    else:
+        pr = [B.Pressure()]
+    gpu = B.Model(res, B.Domain(dims, bnd(lo), bnd(hi)), pr)
+    st = {f: g[f"state_{f}"] for f in gpu.fields}
+    gpu.set_state(**st)
+    got = gpu.rates()
+    for f in gpu.fields:
+        assert relerr(got[f], g[f"rates_{f}"]) < RTOL_RHS64
+    gpu.project()
+    got = gpu.state
+    for f in gpu.fields:
+        assert relerr(got[f], g[f"projected_{f}"]) < RTOL_RHS64
+    if dt > 0:
+        gpu.set_state(**st)
+        B.evolve(gpu, dt, dt)
+        got = gpu.state
+        for f in gpu.fields:
+            assert relerr(got[f], g[f"step_{f}"]) < RTOL64

@@ -298,3 +298,24 @@ def test_divergence_error():
     m.state["vel1"][1, 1, 2] = np.nan
     with pytest.raises(FloatingPointError, match="diverged"):
         O.evolve(m, 1e-3, 1e-3)
+
+
+# -- golden vectors (tests/golden/make_golden.py) ------------------------------
+@pytest.mark.parametrize("name", ["les_12x14x16", "channel_8x8x6", "pressure_4x4x16"])
+def test_golden_vectors_reproduce(name):
+    """The oracle still produces the committed float64 vectors (regression anchor)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as G
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", f"{name}.npz"))
+    m, dt = G.build(name)
+    for f in m.fields:
+        m.state[f][...] = g[f"state_{f}"]
+    rates = {f: np.zeros_like(a) for f, a in m.state.items()}
+    O.compute_rates(m, rates, m.state)
+    for f in m.fields:
+        assert np.allclose(rates[f], g[f"rates_{f}"], rtol=0, atol=1e-13 * np.abs(g[f"rates_{f}"]).max())
+    O.apply_projections(m, m.state)
+    for f in m.fields:
+        assert np.allclose(m.state[f], g[f"projected_{f}"], rtol=0, atol=1e-13 * np.abs(g[f"projected_{f}"]).max())
