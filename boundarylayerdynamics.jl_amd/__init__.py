@@ -8,7 +8,7 @@ top of it.  Build the library with ``build.build()``.
 from .model import (  # noqa: F401
     Domain, SmoothWall, RoughWall, FreeSlipBoundary, CustomBoundary, SinusoidalMapping,
     MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, StaticSmagorinskyModel, SSPRK33,
-    incompressible_flow, closedchannelflow, openchannelflow, Model, evolve, pdsize,
+    incompressible_flow, closedchannelflow, openchannelflow, Model, Ensemble, evolve, pdsize,
 )
 from . import _lib  # noqa: F401
 from .ldlt import BatchLDLt  # noqa: F401

@@ -23,7 +23,8 @@ KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "
                 "r2c_x2", "stage_project", "check_finite")
 
 # every symbol include/bld.h declares
-EXPORTS = ("bld_create", "bld_destroy", "bld_last_error", "bld_set_state", "bld_get_state", "bld_rates",
+EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_size", "bld_select_instance",
+           "bld_autotune", "bld_tuning", "bld_last_error", "bld_set_state", "bld_get_state", "bld_rates",
            "bld_project", "bld_init_problem", "bld_step", "bld_step_graph", "bld_set_field_physical",
            "bld_get_field_physical", "bld_time_steps", "bld_set_profiling", "bld_kernel_times",
            "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error")
@@ -70,6 +71,11 @@ def load():
     sig = {
         "bld_create": ([ctypes.POINTER(BldConfig), ctypes.POINTER(P)], ctypes.c_int),
         "bld_destroy": ([P], None),
+        "bld_ensemble_create": ([ctypes.c_int, ctypes.POINTER(BldConfig), ctypes.POINTER(P)], ctypes.c_int),
+        "bld_ensemble_size": ([P], ctypes.c_int),
+        "bld_select_instance": ([P, ctypes.c_int], ctypes.c_int),
+        "bld_autotune": ([P, ctypes.c_int], ctypes.c_int),
+        "bld_tuning": ([P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), D], ctypes.c_int),
         "bld_last_error": ([P], ctypes.c_char_p),
         "bld_set_state": ([P, D, D, D], ctypes.c_int),
         "bld_get_state": ([P, D, D, D], ctypes.c_int),

@@ -112,10 +112,11 @@ struct LaunchEntry {
     int batch;        // 1D transforms per block
 };
 
-constexpr int kMaxEntries = 8;
+// Block ranges of one multi-entry launch; the entries live in device memory so
+// one launch can cover every space (or instance) of an ensemble.
 struct LaunchTable {
     int n;
-    LaunchEntry e[kMaxEntries];
+    const LaunchEntry *e;
 };
 
 }  // namespace bld

@@ -120,6 +120,8 @@ static void upload(DevBuf &d, const std::vector<U> &h) {
 // ---------------------------------------------------------------------------
 struct EngineBase {
     virtual ~EngineBase() = default;
+    virtual int size() const = 0;
+    virtual void select(int i) = 0;
     virtual void set_state(const double *const f[3]) = 0;
     virtual void get_state(double *const f[3]) = 0;
     virtual int rates(double *const f[3]) = 0;
@@ -131,6 +133,8 @@ struct EngineBase {
     virtual double time_steps(double dt, int64_t n, bool graph) = 0;
     virtual void set_profiling(bool on) = 0;
     virtual void kernel_time(int kind, double *ms, int64_t *count) = 0;
+    virtual void autotune(int reps) = 0;
+    virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
 };
 
 enum KernelKind {
@@ -138,42 +142,114 @@ enum KernelKind {
     KK_STAGE, KK_CHECK, KK_COUNT
 };
 
+// One model (one bld_config): its device state, constants and descriptors.
+template <typename T>
+struct Instance {
+    bld_config cfg{};
+    int K1 = 0, K2 = 0, nk = 0, n3 = 0;
+    bool mixed = false;
+    std::vector<double> alpha_c, alpha_i, lc, li;
+    DevBuf d_u[3], d_ut[3], d_du[3], d_u00[3], d_ut00[3], d_du00[3];
+    DevBuf d_D1, d_D2, d_DD1, d_DD2, d_ac, d_ai, d_lc, d_li, d_gamma, d_gamma0, d_pbuf, d_xfer_phys;
+    std::vector<std::unique_ptr<DevBuf>> owned;
+    int adv = -1, sgs = -1, xfer = -1;  // indices into the engine's space array
+    LinDesc<T> lin{};
+    ProjDesc<T> proj{};
+    ProjDesc<double> proj00{};
+    bool sequential_stage = false;  // very tall columns
+
+    size_t nC() const { return size_t(nk) * n3; }
+    size_t nI() const { return size_t(nk) * (n3 - 1); }
+    size_t nf(int f) const { return f < 2 ? nC() : nI(); }
+    DevBuf &own(size_t bytes) {
+        owned.emplace_back(new DevBuf());
+        owned.back()->alloc(bytes);
+        return *owned.back();
+    }
+};
+
+static void check_config(const bld_config &c) {
+    if (c.abi_version != BLD_ABI_VERSION) throw BldError{BLD_EARG, "ABI version mismatch"};
+    if (c.k1max < 0 || c.k2max < 0) throw BldError{BLD_EARG, "negative wavenumber range"};
+    if (c.n3 < 3) throw BldError{BLD_EARG, "Due to boundary stencils, the grid needs at least 3 vertical layers"};
+    if (!c.alpha_c || !c.alpha_i) throw BldError{BLD_EARG, "missing vertical scale factors"};
+    for (int f = 0; f < 3; ++f)
+        for (int s = 0; s < 2; ++s)
+            if (c.bc[f][s].kind != BLD_BC_DIRICHLET && c.bc[f][s].kind != BLD_BC_NEUMANN)
+                throw BldError{BLD_EARG, "unknown boundary condition kind"};
+    const bool w_dir = c.bc[2][0].kind == BLD_BC_DIRICHLET && c.bc[2][1].kind == BLD_BC_DIRICHLET;
+    if (c.diffusion[2] && !w_dir)
+        throw BldError{BLD_EUNSUPPORTED, "diffusion of vel3 supports Dirichlet boundary conditions only"};
+    if (c.pressure) {
+        if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "The pressure solver only supports Dirichlet BCs"};
+        if (c.bc[2][0].value != c.bc[2][1].value)
+            throw BldError{BLD_EUNSUPPORTED, "The mean flow must be equal at both vertical boundaries."};
+    }
+    if (c.advection) {
+        if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "advection requires Dirichlet boundary conditions for vel3"};
+        if (c.bc[2][0].value != 0 || c.bc[2][1].value != 0)
+            throw BldError{BLD_EUNSUPPORTED, "Advection for non-zero w at boundary not implemented"};
+        if (c.adv_n1 < 1 || c.adv_n2 < 1) throw BldError{BLD_EARG, "invalid advection grid"};
+    }
+    if (c.sgs) {
+        if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "the SGS model requires Dirichlet boundary conditions for vel3"};
+        if (!c.sgs_lc || !c.sgs_li) throw BldError{BLD_EARG, "missing SGS length scales"};
+        if (c.sgs_n1 < 1 || c.sgs_n2 < 1) throw BldError{BLD_EARG, "invalid SGS grid"};
+    }
+}
+
 template <typename T>
 class Engine : public EngineBase {
    public:
-    explicit Engine(const bld_config &c) : cfg(c) { build(); }
+    Engine(int n, const bld_config *cfgs) {
+        if (n < 1) throw BldError{BLD_EARG, "an engine needs at least one model"};
+        for (int i = 0; i < n; ++i) check_config(cfgs[i]);
+        for (int i = 0; i < n; ++i)
+            if (cfgs[i].precision != cfgs[0].precision)
+                throw BldError{BLD_EARG, "all models of an ensemble share one precision"};
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        d_flag.alloc(sizeof(int));
+        HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
+        for (int i = 0; i < n; ++i) {
+            inst.emplace_back(new Instance<T>());
+            build_instance(*inst.back(), cfgs[i]);
+        }
+        upload(d_tw, tw.host);
+        for (auto &s : spaces) s.tw = d_tw.as<cplx<T>>();
+        d_spaces.alloc(spaces.size() * sizeof(Space<T>));
+        sync_spaces();
+        build_plans();
+        HIPCHECK(hipDeviceSynchronize());
+    }
     ~Engine() override {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (auto &ev : events) (void)hipEventDestroy(ev);
         if (st) (void)hipStreamDestroy(st);
     }
 
-    // -- construction -----------------------------------------------------
-    bld_config cfg;
-    int K1 = 0, K2 = 0, nk = 0, n3 = 0;
-    bool mixed = false;
     hipStream_t st = nullptr;
-    std::vector<double> alpha_c, alpha_i, lc, li;
-    DevBuf d_u[3], d_ut[3], d_du[3], d_u00[3], d_ut00[3], d_du00[3];
-    DevBuf d_D1, d_D2, d_DD1, d_DD2, d_ac, d_ai, d_lc, d_li, d_gamma, d_pbuf, d_flag, d_tw, d_spaces;
+    std::vector<std::unique_ptr<Instance<T>>> inst;
+    int sel_inst = 0;
     TwiddleSet<T> tw;
-    std::vector<Space<T>> spaces;  // host copies
-    std::vector<std::unique_ptr<DevBuf>> owned;
-    int adv_idx = -1, sgs_idx = -1, xfer_idx = -1;
-    DevBuf d_xfer_phys;
-    LinDesc<T> lin{};
-    ProjDesc<T> proj{};
-    ProjDesc<double> proj00{};  // float64 (0,0) column of the mixed form
-    DevBuf d_gamma0;
-    // launch tables of the right-hand side
+    DevBuf d_tw, d_spaces, d_flag, d_lin, d_proj, d_proj00;
+    std::vector<Space<T>> spaces;
+
+    // One launch: its entries (host copy + device copy), grid size and LDS.
     struct Pass {
-        LaunchTable lt{};
+        std::vector<LaunchEntry> ent;
+        std::vector<int> kind;  // pass kind of each entry (transform passes)
+        DevBuf dev;
         int nblocks = 0;
         size_t smem = 0;
+        LaunchTable lt() const { return LaunchTable{int(ent.size()), dev.template as<LaunchEntry>()}; }
     };
-    Pass rhs_pass[4];          // combined over adv+sgs for c2r_x2, c2r_x1, r2c_x1
-    Pass r2c_x2_pass[2];       // adv, sgs separately (both accumulate into du)
-    Pass xfer_pass[4];
+    Pass tp[4];          // transform passes over all spaces (r2c_x2 only advection spaces)
+    Pass tp_r2c_x2_sgs;  // r2c_x2 of the SGS spaces (accumulates into the same rates)
+    Pass pp[3];          // physics: advection products, SGS viscosity, SGS stress
+    Pass lin_pass, stage_pass, stage00_pass, xfer[4];
+    int stage_CL = 4;
+    // tuning results: per transform pass and entry
+    std::vector<double> tuned_us[5];
     // graph
     hipGraphExec_t graph_exec = nullptr;
     double graph_dt = 0;
@@ -184,247 +260,208 @@ class Engine : public EngineBase {
     double kt_ms[KK_COUNT] = {};
     int64_t kt_n[KK_COUNT] = {};
 
-    size_t nC() const { return size_t(nk) * n3; }
-    size_t nI() const { return size_t(nk) * (n3 - 1); }
-    size_t nf(int f) const { return f < 2 ? nC() : nI(); }
-
-    DevBuf &own(size_t bytes) {
-        owned.emplace_back(new DevBuf());
-        owned.back()->alloc(bytes);
-        return *owned.back();
-    }
-
-    void check_config() {
-        const bld_config &c = cfg;
-        if (c.abi_version != BLD_ABI_VERSION) throw BldError{BLD_EARG, "ABI version mismatch"};
-        if (c.k1max < 0 || c.k2max < 0) throw BldError{BLD_EARG, "negative wavenumber range"};
-        if (c.n3 < 3) throw BldError{BLD_EARG, "Due to boundary stencils, the grid needs at least 3 vertical layers"};
-        if (!c.alpha_c || !c.alpha_i) throw BldError{BLD_EARG, "missing vertical scale factors"};
-        for (int f = 0; f < 3; ++f)
-            for (int s = 0; s < 2; ++s)
-                if (c.bc[f][s].kind != BLD_BC_DIRICHLET && c.bc[f][s].kind != BLD_BC_NEUMANN)
-                    throw BldError{BLD_EARG, "unknown boundary condition kind"};
-        const bool w_dir = c.bc[2][0].kind == BLD_BC_DIRICHLET && c.bc[2][1].kind == BLD_BC_DIRICHLET;
-        if (c.diffusion[2] && !w_dir)
-            throw BldError{BLD_EUNSUPPORTED, "diffusion of vel3 supports Dirichlet boundary conditions only"};
-        if (c.pressure) {
-            if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "The pressure solver only supports Dirichlet BCs"};
-            if (c.bc[2][0].value != c.bc[2][1].value)
-                throw BldError{BLD_EUNSUPPORTED, "The mean flow must be equal at both vertical boundaries."};
-        }
-        if (c.advection) {
-            if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "advection requires Dirichlet boundary conditions for vel3"};
-            if (c.bc[2][0].value != 0 || c.bc[2][1].value != 0)
-                throw BldError{BLD_EUNSUPPORTED, "Advection for non-zero w at boundary not implemented"};
-            if (c.adv_n1 < 1 || c.adv_n2 < 1) throw BldError{BLD_EARG, "invalid advection grid"};
-        }
-        if (c.sgs) {
-            if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "the SGS model requires Dirichlet boundary conditions for vel3"};
-            if (!c.sgs_lc || !c.sgs_li) throw BldError{BLD_EARG, "missing SGS length scales"};
-            if (c.sgs_n1 < 1 || c.sgs_n2 < 1) throw BldError{BLD_EARG, "invalid SGS grid"};
-        }
-    }
-
-    void build() {
-        check_config();
-        mixed = cfg.precision == BLD_FP32_MIXED;
-        K1 = cfg.k1max + 1;
-        K2 = 2 * cfg.k2max + 1;
-        nk = K1 * K2;
-        n3 = cfg.n3;
-        alpha_c.assign(cfg.alpha_c, cfg.alpha_c + n3);
-        alpha_i.assign(cfg.alpha_i, cfg.alpha_i + n3 + 1);
-        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        for (int f = 0; f < 3; ++f) {
-            d_u[f].alloc(nf(f) * sizeof(cplx<T>));
-            d_ut[f].alloc(nf(f) * sizeof(cplx<T>));
-            d_du[f].alloc(nf(f) * sizeof(cplx<T>));
-            HIPCHECK(hipMemset(d_u[f].p, 0, d_u[f].bytes));
-            HIPCHECK(hipMemset(d_ut[f].p, 0, d_ut[f].bytes));
-            HIPCHECK(hipMemset(d_du[f].p, 0, d_du[f].bytes));
-            if (mixed) {
-                for (DevBuf *b : {&d_u00[f], &d_ut00[f], &d_du00[f]}) {
-                    b->alloc(size_t(n3) * sizeof(cplx<double>));
-                    HIPCHECK(hipMemset(b->p, 0, b->bytes));
-                }
-            }
-        }
-        // derivative factors (Domains.jl:285-288, derivatives.jl:21-27)
-        const double two_pi = 2 * M_PI;
-        std::vector<double> D1(K1), D2(K2), DD1(K1), DD2(K2);
-        for (int i = 0; i < K1; ++i) {
-            D1[i] = i * (two_pi / cfg.L1);
-            DD1[i] = -double(i) * i * ((two_pi / cfg.L1) * (two_pi / cfg.L1));
-        }
-        for (int j = 0; j < K2; ++j) {
-            int k2 = j <= cfg.k2max ? j : j - K2;
-            D2[j] = k2 * (two_pi / cfg.L2);
-            DD2[j] = -double(k2) * k2 * ((two_pi / cfg.L2) * (two_pi / cfg.L2));
-        }
-        upload(d_D1, D1);
-        upload(d_D2, D2);
-        upload(d_DD1, DD1);
-        upload(d_DD2, DD2);
-        upload(d_ac, alpha_c);
-        upload(d_ai, alpha_i);
-        d_flag.alloc(sizeof(int));
-        HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
-        if (cfg.pressure) build_pressure();
-        build_linear();
-        build_proj();
-        if (cfg.advection) adv_idx = add_space(SK_ADVECTION, cfg.adv_n1, cfg.adv_n2);
-        if (cfg.sgs) {
-            lc.assign(cfg.sgs_lc, cfg.sgs_lc + n3);
-            li.assign(cfg.sgs_li, cfg.sgs_li + n3 - 1);
-            upload(d_lc, lc);
-            upload(d_li, li);
-            sgs_idx = add_space(SK_SGS, cfg.sgs_n1, cfg.sgs_n2);
-        }
-        xfer_idx = add_space(SK_TRANSFER, 2 + 2 * cfg.k1max, 2 + 2 * cfg.k2max);
-        upload(d_tw, tw.host);
-        for (auto &s : spaces) s.tw = d_tw.as<cplx<T>>();
-        d_spaces.alloc(spaces.size() * sizeof(Space<T>));
-        sync_spaces();
-        build_passes();
-        HIPCHECK(hipDeviceSynchronize());
+    Instance<T> &cur() { return *inst[sel_inst]; }
+    int size() const override { return int(inst.size()); }
+    void select(int i) override {
+        if (i < 0 || i >= int(inst.size())) throw BldError{BLD_EARG, "instance index out of range"};
+        sel_inst = i;
     }
 
     void sync_spaces() {
         HIPCHECK(hipMemcpy(d_spaces.p, spaces.data(), spaces.size() * sizeof(Space<T>), hipMemcpyHostToDevice));
     }
 
+    // -- construction of one model ------------------------------------------
+    void build_instance(Instance<T> &I, const bld_config &c) {
+        I.cfg = c;
+        I.mixed = c.precision == BLD_FP32_MIXED;
+        I.K1 = c.k1max + 1;
+        I.K2 = 2 * c.k2max + 1;
+        I.nk = I.K1 * I.K2;
+        I.n3 = c.n3;
+        I.alpha_c.assign(c.alpha_c, c.alpha_c + I.n3);
+        I.alpha_i.assign(c.alpha_i, c.alpha_i + I.n3 + 1);
+        for (int f = 0; f < 3; ++f) {
+            for (DevBuf *b : {&I.d_u[f], &I.d_ut[f], &I.d_du[f]}) {
+                b->alloc(I.nf(f) * sizeof(cplx<T>));
+                HIPCHECK(hipMemset(b->p, 0, b->bytes));
+            }
+            if (I.mixed) {
+                for (DevBuf *b : {&I.d_u00[f], &I.d_ut00[f], &I.d_du00[f]}) {
+                    b->alloc(size_t(I.n3) * sizeof(cplx<double>));
+                    HIPCHECK(hipMemset(b->p, 0, b->bytes));
+                }
+            }
+        }
+        // derivative factors (Domains.jl:285-288, derivatives.jl:21-27)
+        const double two_pi = 2 * M_PI;
+        std::vector<double> D1(I.K1), D2(I.K2), DD1(I.K1), DD2(I.K2);
+        for (int i = 0; i < I.K1; ++i) {
+            D1[i] = i * (two_pi / c.L1);
+            DD1[i] = -double(i) * i * ((two_pi / c.L1) * (two_pi / c.L1));
+        }
+        for (int j = 0; j < I.K2; ++j) {
+            const int k2 = j <= c.k2max ? j : j - I.K2;
+            D2[j] = k2 * (two_pi / c.L2);
+            DD2[j] = -double(k2) * k2 * ((two_pi / c.L2) * (two_pi / c.L2));
+        }
+        upload(I.d_D1, D1);
+        upload(I.d_D2, D2);
+        upload(I.d_DD1, DD1);
+        upload(I.d_DD2, DD2);
+        upload(I.d_ac, I.alpha_c);
+        upload(I.d_ai, I.alpha_i);
+        if (c.pressure) build_pressure(I);
+        build_linear(I);
+        build_proj(I);
+        if (c.advection) I.adv = add_space(I, SK_ADVECTION, c.adv_n1, c.adv_n2);
+        if (c.sgs) {
+            I.lc.assign(c.sgs_lc, c.sgs_lc + I.n3);
+            I.li.assign(c.sgs_li, c.sgs_li + I.n3 - 1);
+            upload(I.d_lc, I.lc);
+            upload(I.d_li, I.li);
+            I.sgs = add_space(I, SK_SGS, c.sgs_n1, c.sgs_n2);
+        }
+        I.xfer = add_space(I, SK_TRANSFER, 2 + 2 * c.k1max, 2 + 2 * c.k2max);
+    }
+
     // pressure factorisation (pressure.jl:231-296, 86-95) in float64
-    void build_pressure() {
+    void build_pressure(Instance<T> &I) {
+        const int n3 = I.n3, nk = I.nk, K1 = I.K1, K2 = I.K2;
+        const auto &ai = I.alpha_i, &ac = I.alpha_c;
         std::vector<double> d3g3(n3, 0.0);
-        for (int i = 1; i < n3; ++i) d3g3[i] -= alpha_i[i];
-        for (int i = 0; i < n3 - 1; ++i) d3g3[i] -= alpha_i[i + 1];
-        const double s1 = 1 / cfg.L1, s2 = 1 / cfg.L2;
+        for (int i = 1; i < n3; ++i) d3g3[i] -= ai[i];
+        for (int i = 0; i < n3 - 1; ++i) d3g3[i] -= ai[i + 1];
+        const double s1 = 1 / I.cfg.L1, s2 = 1 / I.cfg.L2;
         std::vector<double> gamma(size_t(n3) * nk), dv(n3);
         for (int m = 0; m < nk; ++m) {
             const int k1 = m % K1, j = m / K1;
-            const int k2 = j <= cfg.k2max ? j : j - K2;
+            const int k2 = j <= I.cfg.k2max ? j : j - K2;
             if (k1 == 0 && k2 == 0) {
                 dv = d3g3;
-                dv[n3 - 1] = -3 * alpha_i[n3 - 1];
+                dv[n3 - 1] = -3 * ai[n3 - 1];
             } else {
                 const double shift = 4 * M_PI * M_PI * (double(k1) * k1 * s1 * s1 + double(k2) * k2 * s2 * s2);
-                for (int i = 0; i < n3; ++i) dv[i] = d3g3[i] - shift / alpha_c[i];
+                for (int i = 0; i < n3; ++i) dv[i] = d3g3[i] - shift / ac[i];
             }
             double g = 1 / dv[0];
             gamma[m] = g;
             for (int i = 1; i < n3; ++i) {
-                const double b = alpha_i[i];
+                const double b = ai[i];
                 g = 1 / (dv[i] - g * (b * b));
                 gamma[size_t(i) * nk + m] = g;
             }
         }
-        upload(d_gamma, gamma);
+        upload(I.d_gamma, gamma);
         std::vector<double> gamma0(n3);
         for (int i = 0; i < n3; ++i) gamma0[i] = gamma[size_t(i) * nk];
-        upload(d_gamma0, gamma0);
-        d_pbuf.alloc(size_t(n3) * nk * sizeof(cplx<double>));
+        upload(I.d_gamma0, gamma0);
+        I.d_pbuf.alloc(size_t(n3) * nk * sizeof(cplx<double>));
     }
 
-    void build_linear() {
-        LinDesc<T> &L = lin;
-        L.K1 = K1;
-        L.K2 = K2;
-        L.nk = nk;
-        L.n3 = n3;
-        L.DD1 = d_DD1.as<double>();
-        L.DD2 = d_DD2.as<double>();
-        L.alpha_c = d_ac.as<double>();
-        L.alpha_i = d_ai.as<double>();
+    void build_linear(Instance<T> &I) {
+        LinDesc<T> &L = I.lin;
+        L.K1 = I.K1;
+        L.K2 = I.K2;
+        L.nk = I.nk;
+        L.n3 = I.n3;
+        L.DD1 = I.d_DD1.template as<double>();
+        L.DD2 = I.d_DD2.template as<double>();
+        L.alpha_c = I.d_ac.template as<double>();
+        L.alpha_i = I.d_ai.template as<double>();
         for (int f = 0; f < 3; ++f) {
-            L.has_diff[f] = cfg.diffusion[f];
-            L.nu[f] = cfg.diffusivity[f];
-            L.src[f] = cfg.source[f];
+            L.has_diff[f] = I.cfg.diffusion[f];
+            L.nu[f] = I.cfg.diffusivity[f];
+            L.src[f] = I.cfg.source[f];
             for (int s = 0; s < 2; ++s) {
-                L.bc_kind[f][s] = cfg.bc[f][s].kind;
-                L.bc_val[f][s] = cfg.bc[f][s].value;
-                L.bc_diff[f][s] = cfg.bc[f][s].difference;
+                L.bc_kind[f][s] = I.cfg.bc[f][s].kind;
+                L.bc_val[f][s] = I.cfg.bc[f][s].value;
+                L.bc_diff[f][s] = I.cfg.bc[f][s].difference;
             }
-            L.state[0][f] = d_u[f].as<cplx<T>>();
-            L.state[1][f] = d_ut[f].as<cplx<T>>();
-            L.shadow[0][f] = mixed ? d_u00[f].as<cplx<double>>() : nullptr;
-            L.shadow[1][f] = mixed ? d_ut00[f].as<cplx<double>>() : nullptr;
-            L.du[f] = d_du[f].as<cplx<T>>();
-            L.du00[f] = mixed ? d_du00[f].as<cplx<double>>() : nullptr;
+            L.state[0][f] = I.d_u[f].template as<cplx<T>>();
+            L.state[1][f] = I.d_ut[f].template as<cplx<T>>();
+            L.shadow[0][f] = I.mixed ? I.d_u00[f].template as<cplx<double>>() : nullptr;
+            L.shadow[1][f] = I.mixed ? I.d_ut00[f].template as<cplx<double>>() : nullptr;
+            L.du[f] = I.d_du[f].template as<cplx<T>>();
+            L.du00[f] = I.mixed ? I.d_du00[f].template as<cplx<double>>() : nullptr;
         }
     }
 
-    void build_proj() {
-        ProjDesc<T> &D = proj;
-        D.K1 = K1;
-        D.K2 = K2;
-        D.nk = nk;
-        D.n3 = n3;
-        D.has_pressure = cfg.pressure;
-        D.D1 = d_D1.as<double>();
-        D.D2 = d_D2.as<double>();
-        D.alpha_c = d_ac.as<double>();
-        D.alpha_i = d_ai.as<double>();
-        D.gamma = cfg.pressure ? d_gamma.as<double>() : nullptr;
-        D.wbc[0] = cfg.bc[2][0].value;
-        D.wbc[1] = cfg.bc[2][1].value;
-        D.pbuf = cfg.pressure ? d_pbuf.as<cplx<double>>() : nullptr;
+    void build_proj(Instance<T> &I) {
+        ProjDesc<T> &D = I.proj;
+        D.K1 = I.K1;
+        D.K2 = I.K2;
+        D.nk = I.nk;
+        D.n3 = I.n3;
+        D.has_pressure = I.cfg.pressure;
+        D.D1 = I.d_D1.template as<double>();
+        D.D2 = I.d_D2.template as<double>();
+        D.alpha_c = I.d_ac.template as<double>();
+        D.alpha_i = I.d_ai.template as<double>();
+        D.gamma = I.cfg.pressure ? I.d_gamma.template as<double>() : nullptr;
+        D.wbc[0] = I.cfg.bc[2][0].value;
+        D.wbc[1] = I.cfg.bc[2][1].value;
+        D.pbuf = I.cfg.pressure ? I.d_pbuf.template as<cplx<double>>() : nullptr;
         for (int f = 0; f < 3; ++f) {
-            D.u[f] = d_u[f].as<cplx<T>>();
-            D.ut[f] = d_ut[f].as<cplx<T>>();
-            D.du[f] = d_du[f].as<cplx<T>>();
-            D.u00[f] = mixed ? d_u00[f].as<cplx<double>>() : nullptr;
-            D.ut00[f] = mixed ? d_ut00[f].as<cplx<double>>() : nullptr;
-            D.du00[f] = mixed ? d_du00[f].as<cplx<double>>() : nullptr;
+            D.u[f] = I.d_u[f].template as<cplx<T>>();
+            D.ut[f] = I.d_ut[f].template as<cplx<T>>();
+            D.du[f] = I.d_du[f].template as<cplx<T>>();
+            D.u00[f] = I.mixed ? I.d_u00[f].template as<cplx<double>>() : nullptr;
+            D.ut00[f] = I.mixed ? I.d_ut00[f].template as<cplx<double>>() : nullptr;
+            D.du00[f] = I.mixed ? I.d_du00[f].template as<cplx<double>>() : nullptr;
         }
         D.flag = d_flag.as<int>();
         D.ext_stride = 0;
-        if (!mixed) return;
-        ProjDesc<double> &Z = proj00;  // one column (k1 = k2 = 0) over the float64 shadows
+        D.skip0 = I.mixed ? 1 : 0;
+        if (!I.mixed) return;
+        ProjDesc<double> &Z = I.proj00;  // one column (k1 = k2 = 0) over the float64 shadows
         Z.K1 = Z.K2 = Z.nk = 1;
-        Z.n3 = n3;
-        Z.has_pressure = cfg.pressure;
-        Z.D1 = d_D1.as<double>();  // D1[0] = D2[0] = 0
-        Z.D2 = d_D2.as<double>();
-        Z.alpha_c = d_ac.as<double>();
-        Z.alpha_i = d_ai.as<double>();
-        Z.gamma = cfg.pressure ? d_gamma0.as<double>() : nullptr;
+        Z.n3 = I.n3;
+        Z.has_pressure = I.cfg.pressure;
+        Z.D1 = I.d_D1.template as<double>();  // D1[0] = D2[0] = 0
+        Z.D2 = I.d_D2.template as<double>();
+        Z.alpha_c = D.alpha_c;
+        Z.alpha_i = D.alpha_i;
+        Z.gamma = I.cfg.pressure ? I.d_gamma0.template as<double>() : nullptr;
         Z.wbc[0] = D.wbc[0];
         Z.wbc[1] = D.wbc[1];
         Z.pbuf = nullptr;
         for (int f = 0; f < 3; ++f) {
-            Z.u[f] = d_u00[f].as<cplx<double>>();
-            Z.ut[f] = d_ut00[f].as<cplx<double>>();
-            Z.du[f] = d_du00[f].as<cplx<double>>();
+            Z.u[f] = I.d_u00[f].template as<cplx<double>>();
+            Z.ut[f] = I.d_ut00[f].template as<cplx<double>>();
+            Z.du[f] = I.d_du00[f].template as<cplx<double>>();
             Z.u00[f] = Z.ut00[f] = Z.du00[f] = nullptr;
-            Z.du_add[f] = reinterpret_cast<const cplx<float> *>(d_du[f].p);
-            Z.mirror_u[f] = reinterpret_cast<cplx<float> *>(d_u[f].p);
-            Z.mirror_ut[f] = reinterpret_cast<cplx<float> *>(d_ut[f].p);
+            Z.du_add[f] = reinterpret_cast<const cplx<float> *>(I.d_du[f].p);
+            Z.mirror_u[f] = reinterpret_cast<cplx<float> *>(I.d_u[f].p);
+            Z.mirror_ut[f] = reinterpret_cast<cplx<float> *>(I.d_ut[f].p);
         }
         Z.flag = d_flag.as<int>();
-        Z.ext_stride = nk;
+        Z.ext_stride = I.nk;
+        Z.skip0 = 0;
     }
 
-    int add_space(int kind, int N1, int N2) {
+    int add_space(Instance<T> &I, int kind, int N1, int N2) {
+        const int n3 = I.n3;
         Space<T> S;
         std::memset(&S, 0, sizeof(S));
         S.kind = kind;
         S.N1 = N1;
         S.N2 = N2;
-        S.K1 = K1;
-        S.K2 = K2;
-        S.k1m = std::min(N1 / 2 + 1, K1) - 1;
-        S.k2m = (std::min(N2, K2) - 1) / 2;
+        S.K1 = I.K1;
+        S.K2 = I.K2;
+        S.k1m = std::min(N1 / 2 + 1, I.K1) - 1;
+        S.k2m = (std::min(N2, I.K2) - 1) / 2;
         S.nk1 = S.k1m + 1;
         S.n3 = n3;
         S.p1 = make_plan(N1, tw.add(N1));
         S.p2 = make_plan(N2, tw.add(N2));
-        S.D1 = d_D1.as<double>();
-        S.D2 = d_D2.as<double>();
-        S.alpha_c = d_ac.as<double>();
-        S.alpha_i = d_ai.as<double>();
+        S.D1 = I.d_D1.template as<double>();
+        S.D2 = I.d_D2.template as<double>();
+        S.alpha_c = I.d_ac.template as<double>();
+        S.alpha_i = I.d_ai.template as<double>();
         for (int s = 0; s < 2; ++s)
-            for (int f = 0; f < 3; ++f) S.state[s][f] = (s == 0 ? d_u[f] : d_ut[f]).template as<cplx<T>>();
-        for (int f = 0; f < 3; ++f) S.tgt_out[f] = d_du[f].as<cplx<T>>();
+            for (int f = 0; f < 3; ++f) S.state[s][f] = (s == 0 ? I.d_u[f] : I.d_ut[f]).template as<cplx<T>>();
+        for (int f = 0; f < 3; ++f) S.tgt_out[f] = I.d_du[f].template as<cplx<T>>();
         const size_t P = size_t(N1) * N2;
         auto set_terms = [&](std::vector<std::pair<int, int>> terms) {  // (recipe, is_I)
             S.nin = int(terms.size());
@@ -434,7 +471,7 @@ class Engine : public EngineBase {
                 S.in_layers[t] = terms[t].second ? n3 - 1 : n3;
                 S.in_job0[t] = j;
                 j += S.in_layers[t];
-                S.in_phys[t] = own(P * S.in_layers[t] * sizeof(T)).template as<T>();
+                S.in_phys[t] = I.own(P * S.in_layers[t] * sizeof(T)).template as<T>();
             }
             S.in_job0[S.nin] = j;
         };
@@ -445,7 +482,7 @@ class Engine : public EngineBase {
                 S.src_layers[s] = isI[s] ? n3 - 1 : n3;
                 S.src_job0[s] = j;
                 j += S.src_layers[s];
-                S.src_phys[s] = own(P * S.src_layers[s] * sizeof(T)).template as<T>();
+                S.src_phys[s] = I.own(P * S.src_layers[s] * sizeof(T)).template as<T>();
             }
             S.src_job0[S.nsrc] = j;
         };
@@ -484,30 +521,30 @@ class Engine : public EngineBase {
             // du3 += FFT(R3 + D1·T13) + D2·FFT(T23)
             S.tgt_nG[2] = 2; S.tgt_G[2][0] = 2; S.tgt_Gd1[2][0] = 0; S.tgt_G[2][1] = 6; S.tgt_Gd1[2][1] = 1;
             S.tgt_nH[2] = 1; S.tgt_H[2][0] = 7;
-            S.visc_c = own(P * n3 * sizeof(T)).template as<T>();
-            S.visc_i = own(P * (n3 - 1) * sizeof(T)).template as<T>();
-            S.lc = d_lc.as<double>();
-            S.li = d_li.as<double>();
+            S.visc_c = I.own(P * n3 * sizeof(T)).template as<T>();
+            S.visc_i = I.own(P * (n3 - 1) * sizeof(T)).template as<T>();
+            S.lc = I.d_lc.template as<double>();
+            S.li = I.d_li.template as<double>();
             for (int s = 0; s < 2; ++s) {
-                S.wall_rough[s] = cfg.sgs_wall[s] == BLD_WALL_ROUGH;
-                S.fstrain[s] = cfg.sgs_factor_strain[s];
-                S.fstress[s] = cfg.sgs_factor_stress[s];
+                S.wall_rough[s] = I.cfg.sgs_wall[s] == BLD_WALL_ROUGH;
+                S.fstrain[s] = I.cfg.sgs_factor_strain[s];
+                S.fstress[s] = I.cfg.sgs_factor_stress[s];
             }
-            S.wbc[0] = cfg.bc[2][0].value;
-            S.wbc[1] = cfg.bc[2][1].value;
+            S.wbc[0] = I.cfg.bc[2][0].value;
+            S.wbc[1] = I.cfg.bc[2][1].value;
         } else {  // transfer space: one term / one source, set per call
             S.nin = 1;
             S.in_recipe[0] = R_FIELD;
             S.in_layers[0] = n3;
             S.in_job0[0] = 0;
             S.in_job0[1] = n3;
-            d_xfer_phys.alloc(P * n3 * sizeof(T));
-            S.in_phys[0] = d_xfer_phys.as<T>();
+            I.d_xfer_phys.alloc(P * n3 * sizeof(T));
+            S.in_phys[0] = I.d_xfer_phys.template as<T>();
             S.nsrc = 1;
             S.src_layers[0] = n3;
             S.src_job0[0] = 0;
             S.src_job0[1] = n3;
-            S.src_phys[0] = d_xfer_phys.as<T>();
+            S.src_phys[0] = I.d_xfer_phys.template as<T>();
             S.ntgt = 1;
             S.tgt_layers[0] = n3;
             S.tgt_job0[0] = 0;
@@ -518,13 +555,32 @@ class Engine : public EngineBase {
             S.tgt_nH[0] = 0;
         }
         const size_t jobs = std::max<size_t>(S.in_job0[S.nin], S.src_job0[S.nsrc]);
-        S.inter = own(jobs * size_t(N2) * S.nk1 * sizeof(cplx<T>)).template as<cplx<T>>();
+        S.inter = I.own(jobs * size_t(N2) * S.nk1 * sizeof(cplx<T>)).template as<cplx<T>>();
         spaces.push_back(S);
         return int(spaces.size()) - 1;
     }
 
-    // batch sizes: keep one block's LDS at <= 64 KiB so several blocks share a CU
-    static int pick_batch(int n, int bufs, int avail) {
+    // -- launch plans -----------------------------------------------------------
+    // transform length, buffers per transform and available columns/row pairs
+    void pass_geometry(int kind, const Space<T> &S, int &n, int &bufs, int &avail, int &njobs) const {
+        bufs = 2;
+        switch (kind) {
+            case PASS_C2R_X2: n = S.N2; avail = S.nk1; njobs = S.in_job0[S.nin]; break;
+            case PASS_C2R_X1: n = S.N1; avail = (S.N2 + 1) / 2; njobs = S.in_job0[S.nin]; break;
+            case PASS_R2C_X1: n = S.N1; avail = (S.N2 + 1) / 2; njobs = S.src_job0[S.nsrc]; break;
+            default: {
+                n = S.N2;
+                avail = S.nk1;
+                njobs = S.tgt_job0[S.ntgt];
+                bool anyH = false;
+                for (int t = 0; t < S.ntgt; ++t) anyH |= S.tgt_nH[t] > 0;
+                bufs = anyH ? 4 : 2;
+            }
+        }
+    }
+
+    // default batch: keep one block's LDS at <= 64 KiB so several blocks share a CU
+    static int default_batch(int n, int bufs, int avail) {
         const size_t limit = 64 * 1024;
         int b = int(limit / (size_t(bufs) * n * sizeof(cplx<T>)));
         b = std::max(1, std::min(b, 16));
@@ -534,50 +590,134 @@ class Engine : public EngineBase {
         return p2;
     }
 
-    void add_entry(Pass &ps, int kind, int sidx) {
+    void add_entry(Pass &ps, int kind, int sidx, int batch = 0) {
         const Space<T> &S = spaces[sidx];
+        int n, bufs, avail, njobs;
+        pass_geometry(kind, S, n, bufs, avail, njobs);
         LaunchEntry E{};
         E.space = sidx;
-        int n = 0, bufs = 2, avail = 0, njobs = 0;
-        switch (kind) {
-            case PASS_C2R_X2: n = S.N2; avail = S.nk1; njobs = S.in_job0[S.nin]; break;
-            case PASS_C2R_X1: n = S.N1; avail = (S.N2 + 1) / 2; njobs = S.in_job0[S.nin]; break;
-            case PASS_R2C_X1: n = S.N1; avail = (S.N2 + 1) / 2; njobs = S.src_job0[S.nsrc]; break;
-            case PASS_R2C_X2: {
-                n = S.N2;
-                avail = S.nk1;
-                njobs = S.tgt_job0[S.ntgt];
-                bool anyH = false;
-                for (int t = 0; t < S.ntgt; ++t) anyH |= S.tgt_nH[t] > 0;
-                bufs = anyH ? 4 : 2;
-                break;
-            }
-        }
-        E.batch = pick_batch(n, bufs, avail);
+        E.batch = batch > 0 ? batch : default_batch(n, bufs, avail);
         E.blocks_per_job = (avail + E.batch - 1) / E.batch;
         E.njobs = njobs;
         E.block_begin = ps.nblocks;
-        if (ps.lt.n >= kMaxEntries) throw BldError{BLD_EARG, "too many spaces in one launch"};
-        ps.lt.e[ps.lt.n++] = E;
+        ps.ent.push_back(E);
+        ps.kind.push_back(kind);
         ps.nblocks += E.blocks_per_job * njobs;
-        ps.smem = std::max(ps.smem, size_t(bufs) * E.batch * n * sizeof(cplx<T>));
+        const size_t lds = size_t(bufs) * E.batch * n * sizeof(cplx<T>);
+        if (lds > 160 * 1024) throw BldError{BLD_EUNSUPPORTED, "transform length too large for LDS"};
+        ps.smem = std::max(ps.smem, lds);
     }
 
-    void build_passes() {
-        for (int k = 0; k < 4; ++k) {
-            rhs_pass[k] = Pass{};
-            for (int s : {adv_idx, sgs_idx})
-                if (s >= 0 && k != PASS_R2C_X2) add_entry(rhs_pass[k], k, s);
-            xfer_pass[k] = Pass{};
-            add_entry(xfer_pass[k], k, xfer_idx);
+    void finish(Pass &ps) {
+        if (ps.ent.empty()) return;
+        ps.dev.alloc(ps.ent.size() * sizeof(LaunchEntry));
+        HIPCHECK(hipMemcpy(ps.dev.p, ps.ent.data(), ps.ent.size() * sizeof(LaunchEntry), hipMemcpyHostToDevice));
+    }
+
+    void reset(Pass &ps) {
+        ps.ent.clear();
+        ps.kind.clear();
+        ps.nblocks = 0;
+        ps.smem = 0;
+    }
+
+    // batches[k][i]: batch of transform pass k for the i-th model space (0 = default)
+    void build_transform_passes(const std::vector<std::vector<int>> *batches = nullptr) {
+        for (int k = 0; k < 4; ++k) reset(tp[k]);
+        reset(tp_r2c_x2_sgs);
+        std::vector<int> model_spaces;
+        for (auto &I : inst)
+            for (int s : {I->adv, I->sgs})
+                if (s >= 0) model_spaces.push_back(s);
+        for (size_t i = 0; i < model_spaces.size(); ++i) {
+            const int s = model_spaces[i];
+            auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
+            for (int k = 0; k < 3; ++k) add_entry(tp[k], k, s, b(k));
+            if (spaces[s].kind == SK_ADVECTION)
+                add_entry(tp[PASS_R2C_X2], PASS_R2C_X2, s, b(3));
+            else
+                add_entry(tp_r2c_x2_sgs, PASS_R2C_X2, s, b(3));
         }
-        r2c_x2_pass[0] = Pass{};
-        r2c_x2_pass[1] = Pass{};
-        if (adv_idx >= 0) add_entry(r2c_x2_pass[0], PASS_R2C_X2, adv_idx);
-        if (sgs_idx >= 0) add_entry(r2c_x2_pass[1], PASS_R2C_X2, sgs_idx);
-        for (Pass *p : {&rhs_pass[0], &rhs_pass[1], &rhs_pass[2], &rhs_pass[3], &r2c_x2_pass[0], &r2c_x2_pass[1],
-                        &xfer_pass[0], &xfer_pass[1], &xfer_pass[2], &xfer_pass[3]})
-            if (p->smem > 160 * 1024) throw BldError{BLD_EUNSUPPORTED, "transform length too large for LDS"};
+        for (int k = 0; k < 4; ++k) finish(tp[k]);
+        finish(tp_r2c_x2_sgs);
+    }
+
+    void build_plans() {
+        build_transform_passes();
+        // physics and linear terms: 256 threads per block, one entry per space / model
+        for (int k = 0; k < 3; ++k) reset(pp[k]);
+        reset(lin_pass);
+        for (size_t i = 0; i < inst.size(); ++i) {
+            Instance<T> &I = *inst[i];
+            auto add_points = [&](Pass &ps, int sidx) {
+                const Space<T> &S = spaces[sidx];
+                LaunchEntry E{};
+                E.space = sidx;
+                E.block_begin = ps.nblocks;
+                const size_t pts = size_t(S.N1) * S.N2 * S.n3;
+                E.blocks_per_job = int((pts + 255) / 256);
+                E.njobs = 1;
+                ps.ent.push_back(E);
+                ps.nblocks += E.blocks_per_job;
+            };
+            if (I.adv >= 0) add_points(pp[PHYS_ADV], I.adv);
+            if (I.sgs >= 0) {
+                add_points(pp[PHYS_SGS_VISC], I.sgs);
+                add_points(pp[PHYS_SGS_STRESS], I.sgs);
+            }
+            LaunchEntry E{};
+            E.space = int(i);
+            E.block_begin = lin_pass.nblocks;
+            E.blocks_per_job = int((size_t(I.nk) * (3 * size_t(I.n3) - 1) + 255) / 256);
+            E.njobs = 1;
+            lin_pass.ent.push_back(E);
+            lin_pass.nblocks += E.blocks_per_job;
+        }
+        for (int k = 0; k < 3; ++k) finish(pp[k]);
+        finish(lin_pass);
+        std::vector<LinDesc<T>> lins;
+        for (auto &I : inst) lins.push_back(I->lin);
+        upload(d_lin, lins);
+        // chunked stage kernel: CL shared by the launch, NC = pow2 ≥ ⌈n3/CL⌉, MB = 256/NC
+        int n3max = 0;
+        for (auto &I : inst) n3max = std::max(n3max, I->n3);
+        stage_CL = n3max <= 64 ? 4 : (n3max <= 128 ? 8 : 16);
+        reset(stage_pass);
+        reset(stage00_pass);
+        std::vector<ProjDesc<T>> projs;
+        std::vector<ProjDesc<double>> projs00;
+        for (auto &I : inst) {
+            int nc = (I->n3 + stage_CL - 1) / stage_CL, NC = 1;
+            while (NC < nc) NC *= 2;
+            if (NC > 16) {
+                I->sequential_stage = true;
+                continue;
+            }
+            I->proj.MB = 256 / NC;
+            LaunchEntry E{};
+            E.space = int(projs.size());
+            E.block_begin = stage_pass.nblocks;
+            E.blocks_per_job = (I->nk + I->proj.MB - 1) / I->proj.MB;
+            E.njobs = 1;
+            stage_pass.ent.push_back(E);
+            stage_pass.nblocks += E.blocks_per_job;
+            projs.push_back(I->proj);
+            if (I->mixed) {
+                I->proj00.MB = 256 / NC;
+                LaunchEntry Z{};
+                Z.space = int(projs00.size());
+                Z.block_begin = stage00_pass.nblocks;
+                Z.blocks_per_job = 1;
+                Z.njobs = 1;
+                stage00_pass.ent.push_back(Z);
+                stage00_pass.nblocks += 1;
+                projs00.push_back(I->proj00);
+            }
+        }
+        finish(stage_pass);
+        finish(stage00_pass);
+        upload(d_proj, projs);
+        upload(d_proj00, projs00);
     }
 
     // -- launch helpers with optional per-kernel timing ---------------------
@@ -611,40 +751,42 @@ class Engine : public EngineBase {
         ev_kind.clear();
     }
 
-    void run_pass(int kind, const Pass &p, int sel, int kk) {
+    void run_transform(int kind, const Pass &p, int sel, int kk) {
         if (p.nblocks == 0) return;
-        timed(kk, [&] { launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt, p.nblocks, p.smem, sel, st); });
+        timed(kk, [&] { launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt(), p.nblocks, p.smem, sel, st); });
     }
 
-    // compute_rates! of state `sel` (0 = u, 1 = utmp) into du
+    // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du
     void rhs(int sel) {
-        timed(KK_LINEAR, [&] { launch_linear<T>(lin, sel, st); });
-        if (adv_idx < 0 && sgs_idx < 0) return;
-        run_pass(PASS_C2R_X2, rhs_pass[PASS_C2R_X2], sel, KK_C2R_X2);
-        run_pass(PASS_C2R_X1, rhs_pass[PASS_C2R_X1], sel, KK_C2R_X1);
+        timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
+        if (tp[PASS_C2R_X2].nblocks == 0) return;
+        run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
+        run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
         const Space<T> *dsp = d_spaces.as<Space<T>>();
-        if (adv_idx >= 0) {
-            const Space<T> &S = spaces[adv_idx];
-            timed(KK_ADV, [&] { launch_physics<T>(PHYS_ADV, dsp + adv_idx, size_t(S.N1) * S.N2 * n3, st); });
-        }
-        if (sgs_idx >= 0) {
-            const Space<T> &S = spaces[sgs_idx];
-            const size_t np = size_t(S.N1) * S.N2 * n3;
-            timed(KK_SGS_VISC, [&] { launch_physics<T>(PHYS_SGS_VISC, dsp + sgs_idx, np, st); });
-            timed(KK_SGS_STRESS, [&] { launch_physics<T>(PHYS_SGS_STRESS, dsp + sgs_idx, np, st); });
-        }
-        run_pass(PASS_R2C_X1, rhs_pass[PASS_R2C_X1], sel, KK_R2C_X1);
-        run_pass(PASS_R2C_X2, r2c_x2_pass[0], sel, KK_R2C_X2);
-        run_pass(PASS_R2C_X2, r2c_x2_pass[1], sel, KK_R2C_X2);
+        const int kks[3] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS};
+        for (int k = 0; k < 3; ++k)
+            if (pp[k].nblocks) timed(kks[k], [&] { launch_physics<T>(k, dsp, pp[k].lt(), pp[k].nblocks, st); });
+        run_transform(PASS_R2C_X1, tp[PASS_R2C_X1], sel, KK_R2C_X1);
+        run_transform(PASS_R2C_X2, tp[PASS_R2C_X2], sel, KK_R2C_X2);
+        run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
     }
 
     void stage(int s, double dt) {
-        timed(KK_STAGE, [&] { launch_stage_project<T>(proj, s, dt, st, mixed ? &proj00 : nullptr); });
+        timed(KK_STAGE, [&] {
+            launch_stage_chunked<T>(stage_CL, d_proj.as<ProjDesc<T>>(), stage_pass.lt(), stage_pass.nblocks, s, dt,
+                                    st);
+            if (stage00_pass.nblocks)
+                launch_stage_chunked<double>(stage_CL, d_proj00.as<ProjDesc<double>>(), stage00_pass.lt(),
+                                             stage00_pass.nblocks, s, dt, st);
+            for (auto &I : inst)
+                if (I->sequential_stage) launch_stage_sequential<T>(I->proj, s, dt, st);
+        });
     }
 
     void check_du() {
         timed(KK_CHECK, [&] {
-            for (int f = 0; f < 3; ++f) launch_check_finite<T>(d_du[f].p, nf(f), d_flag.as<int>(), st);
+            for (auto &I : inst)
+                for (int f = 0; f < 3; ++f) launch_check_finite<T>(I->d_du[f].p, I->nf(f), d_flag.as<int>(), st);
         });
     }
 
@@ -665,39 +807,40 @@ class Engine : public EngineBase {
     }
     void clear_flag() { HIPCHECK(hipMemsetAsync(d_flag.p, 0, sizeof(int), st)); }
 
-    // -- state transfer -------------------------------------------------------
+    // -- state transfer (selected model) ------------------------------------------
     void set_state(const double *const f[3]) override {
+        Instance<T> &I = cur();
         for (int c = 0; c < 3; ++c) {
             if (!f[c]) continue;
-            const size_t n = nf(c);
+            const size_t n = I.nf(c);
             std::vector<cplx<T>> h(n);
             for (size_t i = 0; i < n; ++i) h[i] = mk(T(f[c][2 * i]), T(f[c][2 * i + 1]));
-            HIPCHECK(hipMemcpy(d_u[c].p, h.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
-            if (mixed) {
-                const int nl = c < 2 ? n3 : n3 - 1;
-                std::vector<cplx<double>> s(n3, mk(0.0, 0.0));
-                for (int l = 0; l < nl; ++l) s[l] = mk(f[c][2 * size_t(l) * nk], f[c][2 * size_t(l) * nk + 1]);
-                HIPCHECK(hipMemcpy(d_u00[c].p, s.data(), n3 * sizeof(cplx<double>), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(I.d_u[c].p, h.data(), n * sizeof(cplx<T>), hipMemcpyHostToDevice));
+            if (I.mixed) {
+                const int nl = c < 2 ? I.n3 : I.n3 - 1;
+                std::vector<cplx<double>> s(I.n3, mk(0.0, 0.0));
+                for (int l = 0; l < nl; ++l) s[l] = mk(f[c][2 * size_t(l) * I.nk], f[c][2 * size_t(l) * I.nk + 1]);
+                HIPCHECK(hipMemcpy(I.d_u00[c].p, s.data(), I.n3 * sizeof(cplx<double>), hipMemcpyHostToDevice));
             }
         }
     }
 
-    void download(const DevBuf (&src)[3], const DevBuf (&s00)[3], bool add00, double *const f[3]) {
+    void download(Instance<T> &I, const DevBuf (&src)[3], const DevBuf (&s00)[3], bool add00, double *const f[3]) {
         for (int c = 0; c < 3; ++c) {
             if (!f[c]) continue;
-            const size_t n = nf(c);
+            const size_t n = I.nf(c);
             std::vector<cplx<T>> h(n);
             HIPCHECK(hipMemcpy(h.data(), src[c].p, n * sizeof(cplx<T>), hipMemcpyDeviceToHost));
             for (size_t i = 0; i < n; ++i) {
                 f[c][2 * i] = double(h[i].x);
                 f[c][2 * i + 1] = double(h[i].y);
             }
-            if (mixed) {
-                const int nl = c < 2 ? n3 : n3 - 1;
-                std::vector<cplx<double>> s(n3);
-                HIPCHECK(hipMemcpy(s.data(), s00[c].p, n3 * sizeof(cplx<double>), hipMemcpyDeviceToHost));
+            if (I.mixed) {
+                const int nl = c < 2 ? I.n3 : I.n3 - 1;
+                std::vector<cplx<double>> s(I.n3);
+                HIPCHECK(hipMemcpy(s.data(), s00[c].p, I.n3 * sizeof(cplx<double>), hipMemcpyDeviceToHost));
                 for (int l = 0; l < nl; ++l) {
-                    const size_t i = size_t(l) * nk;
+                    const size_t i = size_t(l) * I.nk;
                     if (add00) {
                         f[c][2 * i] += s[l].x;
                         f[c][2 * i + 1] += s[l].y;
@@ -710,14 +853,14 @@ class Engine : public EngineBase {
         }
     }
 
-    void get_state(double *const f[3]) override { download(d_u, d_u00, false, f); }
+    void get_state(double *const f[3]) override { download(cur(), cur().d_u, cur().d_u00, false, f); }
 
     int rates(double *const f[3]) override {
         clear_flag();
         rhs(0);
         check_du();
         const int flag = read_flag();
-        download(d_du, d_du00, true, f);
+        download(cur(), cur().d_du, cur().d_du00, true, f);
         return flag ? BLD_EDIVERGED : BLD_OK;
     }
 
@@ -801,38 +944,119 @@ class Engine : public EngineBase {
         *count = kt_n[kind];
     }
 
-    // -- physical-space transfer at the default size ----------------------
+    // -- FFT batch autotuner ------------------------------------------------------
+    // For every transform pass and model space, times each power-of-two batch
+    // (columns or row pairs per block) whose LDS fits, on the live data, and keeps
+    // the fastest.  r2c_x2 accumulates into du, so du is recomputed at the end.
+    double time_entry(int kind, int sidx, int batch, int sel, int reps) {
+        Pass tmp;
+        add_entry(tmp, kind, sidx, batch);
+        finish(tmp);
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        launch_transform<T>(kind, d_spaces.as<Space<T>>(), tmp.lt(), tmp.nblocks, tmp.smem, sel, st);  // warm
+        HIPCHECK(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r)
+            launch_transform<T>(kind, d_spaces.as<Space<T>>(), tmp.lt(), tmp.nblocks, tmp.smem, sel, st);
+        HIPCHECK(hipEventRecord(b, st));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return 1e3 * ms / reps;
+    }
+
+    void autotune(int reps) override {
+        if (reps < 1) reps = 5;
+        std::vector<int> model_spaces;
+        for (auto &I : inst)
+            for (int s : {I->adv, I->sgs})
+                if (s >= 0) model_spaces.push_back(s);
+        std::vector<std::vector<int>> best(4, std::vector<int>(model_spaces.size(), 0));
+        for (int k = 0; k < 5; ++k) tuned_us[k].assign(model_spaces.size(), 0.0);
+        // run the c2r passes once so that every buffer holds live data
+        rhs(0);
+        for (int k = 0; k < 4; ++k) {
+            for (size_t i = 0; i < model_spaces.size(); ++i) {
+                const int s = model_spaces[i];
+                int n, bufs, avail, njobs;
+                pass_geometry(k, spaces[s], n, bufs, avail, njobs);
+                double bt = 1e30;
+                for (int b = 1; b <= 32 && b <= std::max(1, avail); b *= 2) {
+                    if (size_t(bufs) * b * n * sizeof(cplx<T>) > 96 * 1024) break;
+                    const double us = time_entry(k, s, b, 0, reps);
+                    if (us < bt) {
+                        bt = us;
+                        best[k][i] = b;
+                    }
+                }
+                tuned_us[k][i] = bt;
+            }
+        }
+        build_transform_passes(&best);
+        if (graph_exec) {
+            HIPCHECK(hipGraphExecDestroy(graph_exec));
+            graph_exec = nullptr;
+        }
+        rhs(0);  // restore du = f(u)
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    void tuning(int pass, int entry, int *batch, double *us) override {
+        if (pass < 0 || pass > 3) throw BldError{BLD_EARG, "unknown pass"};
+        const Pass &p = pass == 3 ? tp[3] : tp[pass];
+        // entries of pass 3 are split between the advection and SGS launches
+        std::vector<const LaunchEntry *> all;
+        for (auto &e : p.ent) all.push_back(&e);
+        if (pass == 3)
+            for (auto &e : tp_r2c_x2_sgs.ent) all.push_back(&e);
+        if (entry < 0 || entry >= int(all.size())) throw BldError{BLD_EARG, "entry index out of range"};
+        *batch = all[entry]->batch;
+        *us = (pass < 4 && entry < int(tuned_us[pass].size())) ? tuned_us[pass][entry] : 0.0;
+    }
+
+    // -- physical-space transfer at the default size (selected model) -------------
+    void xfer_plan(int k, int sidx) {
+        reset(xfer[k]);
+        add_entry(xfer[k], k, sidx);
+        finish(xfer[k]);
+    }
+
     void set_field_physical(int f, const double *v) override {
         if (f < 0 || f > 2) throw BldError{BLD_EARG, "field index must be 0, 1 or 2"};
-        Space<T> &S = spaces[xfer_idx];
-        const int nl = f < 2 ? n3 : n3 - 1;
+        Instance<T> &I = cur();
+        Space<T> &S = spaces[I.xfer];
+        const int nl = f < 2 ? I.n3 : I.n3 - 1;
         const size_t P = size_t(S.N1) * S.N2;
         std::vector<T> h(P * nl);
         for (size_t i = 0; i < h.size(); ++i) h[i] = T(v[i]);
-        HIPCHECK(hipMemcpy(d_xfer_phys.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(I.d_xfer_phys.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
         S.src_layers[0] = nl;
         S.src_job0[1] = nl;
         S.tgt_layers[0] = nl;
         S.tgt_job0[1] = nl;
-        S.tgt_out[0] = d_u[f].as<cplx<T>>();
+        S.tgt_out[0] = I.d_u[f].template as<cplx<T>>();
         sync_spaces();
-        rebuild_xfer();
-        launch_fill<T>(d_u[f].p, nf(f), st);
-        run_pass(PASS_R2C_X1, xfer_pass[PASS_R2C_X1], 0, KK_R2C_X1);
-        run_pass(PASS_R2C_X2, xfer_pass[PASS_R2C_X2], 0, KK_R2C_X2);
-        if (mixed) {  // float64 mean of each layer for the (0,0) shadow
-            std::vector<cplx<double>> s(n3, mk(0.0, 0.0));
-            std::vector<cplx<T>> h00(nl);
+        xfer_plan(PASS_R2C_X1, I.xfer);
+        xfer_plan(PASS_R2C_X2, I.xfer);
+        launch_fill<T>(I.d_u[f].p, I.nf(f), st);
+        run_transform(PASS_R2C_X1, xfer[PASS_R2C_X1], 0, KK_R2C_X1);
+        run_transform(PASS_R2C_X2, xfer[PASS_R2C_X2], 0, KK_R2C_X2);
+        if (I.mixed) {  // float64 mean of each layer for the (0,0) shadow
+            std::vector<cplx<double>> s(I.n3, mk(0.0, 0.0));
             for (int l = 0; l < nl; ++l) {
                 double sum = 0;
                 for (size_t i = 0; i < P; ++i) sum += v[size_t(l) * P + i];
                 s[l] = mk(sum * (1.0 / double(P)), 0.0);
             }
             HIPCHECK(hipStreamSynchronize(st));
-            HIPCHECK(hipMemcpy(d_u00[f].p, s.data(), n3 * sizeof(cplx<double>), hipMemcpyHostToDevice));
+            HIPCHECK(hipMemcpy(I.d_u00[f].p, s.data(), I.n3 * sizeof(cplx<double>), hipMemcpyHostToDevice));
             for (int l = 0; l < nl; ++l) {
                 cplx<T> z = mk(T(s[l].x), T(0));
-                HIPCHECK(hipMemcpy(d_u[f].as<cplx<T>>() + size_t(l) * nk, &z, sizeof(z), hipMemcpyHostToDevice));
+                HIPCHECK(hipMemcpy(I.d_u[f].template as<cplx<T>>() + size_t(l) * I.nk, &z, sizeof(z),
+                                   hipMemcpyHostToDevice));
             }
         }
         HIPCHECK(hipStreamSynchronize(st));
@@ -840,27 +1064,22 @@ class Engine : public EngineBase {
 
     void get_field_physical(int f, double *v) override {
         if (f < 0 || f > 2) throw BldError{BLD_EARG, "field index must be 0, 1 or 2"};
-        Space<T> &S = spaces[xfer_idx];
-        const int nl = f < 2 ? n3 : n3 - 1;
+        Instance<T> &I = cur();
+        Space<T> &S = spaces[I.xfer];
+        const int nl = f < 2 ? I.n3 : I.n3 - 1;
         S.in_layers[0] = nl;
         S.in_job0[1] = nl;
-        S.in_field[0] = d_u[f].as<cplx<T>>();
+        S.in_field[0] = I.d_u[f].template as<cplx<T>>();
         sync_spaces();
-        rebuild_xfer();
-        run_pass(PASS_C2R_X2, xfer_pass[PASS_C2R_X2], 0, KK_C2R_X2);
-        run_pass(PASS_C2R_X1, xfer_pass[PASS_C2R_X1], 0, KK_C2R_X1);
+        xfer_plan(PASS_C2R_X2, I.xfer);
+        xfer_plan(PASS_C2R_X1, I.xfer);
+        run_transform(PASS_C2R_X2, xfer[PASS_C2R_X2], 0, KK_C2R_X2);
+        run_transform(PASS_C2R_X1, xfer[PASS_C2R_X1], 0, KK_C2R_X1);
         const size_t n = size_t(S.N1) * S.N2 * nl;
         std::vector<T> h(n);
         HIPCHECK(hipStreamSynchronize(st));
-        HIPCHECK(hipMemcpy(h.data(), d_xfer_phys.p, n * sizeof(T), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(h.data(), I.d_xfer_phys.p, n * sizeof(T), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; ++i) v[i] = double(h[i]);
-    }
-
-    void rebuild_xfer() {
-        for (int k = 0; k < 4; ++k) {
-            xfer_pass[k] = Pass{};
-            add_entry(xfer_pass[k], k, xfer_idx);
-        }
     }
 };
 
@@ -987,6 +1206,17 @@ static int guarded(std::string &err, F &&f) {
     }
 }
 
+static bld_engine *make_engine(int n, const bld_config *cfgs) {
+    std::unique_ptr<bld_engine> e(new bld_engine());
+    if (cfgs[0].precision == BLD_FP64)
+        e->impl.reset(new Engine<double>(n, cfgs));
+    else if (cfgs[0].precision == BLD_FP32_MIXED)
+        e->impl.reset(new Engine<float>(n, cfgs));
+    else
+        throw BldError{BLD_EARG, "unknown precision"};
+    return e.release();
+}
+
 extern "C" {
 
 int bld_create(const bld_config *cfg, bld_engine **out) {
@@ -996,14 +1226,19 @@ int bld_create(const bld_config *cfg, bld_engine **out) {
     }
     *out = nullptr;
     return guarded(g_create_error, [&] {
-        std::unique_ptr<bld_engine> e(new bld_engine());
-        if (cfg->precision == BLD_FP64)
-            e->impl.reset(new Engine<double>(*cfg));
-        else if (cfg->precision == BLD_FP32_MIXED)
-            e->impl.reset(new Engine<float>(*cfg));
-        else
-            throw BldError{BLD_EARG, "unknown precision"};
-        *out = e.release();
+        *out = make_engine(1, cfg);
+        return BLD_OK;
+    });
+}
+
+int bld_ensemble_create(int n, const bld_config *cfgs, bld_engine **out) {
+    if (!cfgs || !out || n < 1) {
+        g_create_error = "invalid ensemble arguments";
+        return BLD_EARG;
+    }
+    *out = nullptr;
+    return guarded(g_create_error, [&] {
+        *out = make_engine(n, cfgs);
         return BLD_OK;
     });
 }
@@ -1015,6 +1250,33 @@ const char *bld_last_error(const bld_engine *e) { return e ? e->err.c_str() : g_
 #define ENGINE_CALL(...)                                              \
     if (!e) return BLD_EARG;                                          \
     return guarded(e->err, [&]() -> int { __VA_ARGS__; });
+
+int bld_ensemble_size(bld_engine *e) {
+    if (!e) return BLD_EARG;
+    return e->impl->size();
+}
+
+int bld_select_instance(bld_engine *e, int index) {
+    ENGINE_CALL({
+        e->impl->select(index);
+        return BLD_OK;
+    })
+}
+
+int bld_autotune(bld_engine *e, int reps) {
+    ENGINE_CALL({
+        e->impl->autotune(reps);
+        return BLD_OK;
+    })
+}
+
+int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us) {
+    ENGINE_CALL({
+        if (!batch || !us) throw BldError{BLD_EARG, "null output"};
+        e->impl->tuning(pass, entry, batch, us);
+        return BLD_OK;
+    })
+}
 
 int bld_set_state(bld_engine *e, const double *vel1, const double *vel2, const double *vel3) {
     ENGINE_CALL({

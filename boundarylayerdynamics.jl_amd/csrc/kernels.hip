@@ -25,10 +25,17 @@ namespace bld {
 // helpers
 // ---------------------------------------------------------------------------
 
+// entry of block b (entries sorted by block_begin)
 __device__ __forceinline__ int find_entry(const LaunchTable &lt, int b) {
-    int ei = 0;
-    while (ei + 1 < lt.n && b >= lt.e[ei + 1].block_begin) ++ei;
-    return ei;
+    int lo = 0, hi = lt.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b >= lt.e[mid].block_begin)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 template <typename T>
@@ -349,10 +356,11 @@ __global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spa
 // advection products (momentum_advection.jl:179-206); w = 0 at the walls
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict__ sp) {
-    const Space<T> &S = *sp;
+__global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
     const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
     const int l = int(gid / P);
     const size_t x = gid - size_t(l) * P;
@@ -410,10 +418,11 @@ struct SgsPoint {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restrict__ sp) {
-    const Space<T> &S = *sp;
+__global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
     const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
     const int l = int(gid / P);
     const size_t x = gid - size_t(l) * P;
@@ -460,10 +469,11 @@ __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restric
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ sp) {
-    const Space<T> &S = *sp;
+__global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
     const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
     const int l = int(gid / P);
     const size_t x = gid - size_t(l) * P;
@@ -551,10 +561,12 @@ __device__ __forceinline__ cplx<A> ghost(int kind, double value, double differen
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_linear(LinDesc<T> L, int sel) {
+__global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ descs, LaunchTable lt, int sel) {
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const LinDesc<T> &L = descs[E.space];
     const int nk = L.nk, n3 = L.n3;
     const size_t per_c = size_t(nk) * n3, per_i = size_t(nk) * (n3 - 1);
-    const size_t gid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     int f;
     size_t r;
     if (gid < per_c) {
@@ -638,7 +650,7 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
     const int nk = D.nk, n3 = D.n3;
     const bool mixed00 = (m == 0) && (D.u00[0] != nullptr);
     // input/output selection: stage 0 projects u in place, 1-2 write utmp, 3 writes u
-    cplx<T> **outp = (stage == 1 || stage == 2) ? D.ut : D.u;
+    cplx<T> *const *outp = (stage == 1 || stage == 2) ? D.ut : D.u;
     cplx<double> **outs = (stage == 1 || stage == 2) ? D.ut00 : D.u00;
     bool bad = false;
     auto fin = [&](cplx<T> v) {
@@ -763,19 +775,23 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
 // In the first (forward) / last (backward) chunk this is the sequential
 // recurrence of pressure.jl:126-160 operation for operation.
 template <typename T, int CL>
-__global__ void __launch_bounds__(256) k_stage_chunked(ProjDesc<T> D, int stage, double dt, int MB, int skip0) {
+__global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__restrict__ descs, LaunchTable lt, int stage,
+                                                       double dt) {
     extern __shared__ unsigned char smem_raw[];
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const ProjDesc<T> &D = descs[E.space];
+    const int MB = D.MB, skip0 = D.skip0;
     const int NC = blockDim.x / MB;
     cplx<double> *s_y = reinterpret_cast<cplx<double> *>(smem_raw);  // [NC][MB]
     double *s_p = reinterpret_cast<double *>(s_y + NC * MB);         // [NC][MB]
     cplx<T> *s_w = reinterpret_cast<cplx<T> *>(s_p + NC * MB);      // [NC][MB]
     const int tx = threadIdx.x % MB, ty = threadIdx.x / MB;
-    const int m = blockIdx.x * MB + tx;
+    const int m = (blockIdx.x - E.block_begin) * MB + tx;
     const int nk = D.nk, n3 = D.n3;
     const bool active = m < nk && !(skip0 && m == 0);
     const int l0 = ty * CL;
     const int mm = active ? m : 0;
-    cplx<T> **outp = (stage == 1 || stage == 2) ? D.ut : D.u;
+    cplx<T> *const *outp = (stage == 1 || stage == 2) ? D.ut : D.u;
     bool bad = false;
     cplx<T> c1[CL], c2[CL], c3[CL];
 #pragma unroll
@@ -815,7 +831,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(ProjDesc<T> D, int stage,
                 outp[1][i] = c2[j];
                 if (l < n3 - 1) outp[2][i] = c3[j];
                 if (D.ext_stride) {
-                    cplx<float> **mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+                    cplx<float> *const *mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
                     const size_t e = size_t(l) * D.ext_stride;
                     mir[0][e] = mk(float(c1[j].x), float(c1[j].y));
                     mir[1][e] = mk(float(c2[j].x), float(c2[j].y));
@@ -940,7 +956,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(ProjDesc<T> D, int stage,
         v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
         outp[0][i] = mk(T(v1.x), T(v1.y));
         outp[1][i] = mk(T(v2.x), T(v2.y));
-        cplx<float> **mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+        cplx<float> *const *mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
         const size_t e = size_t(l) * D.ext_stride;
         if (D.ext_stride) {
             mir[0][e] = mk(float(v1.x), float(v1.y));
@@ -989,47 +1005,38 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
 }
 
 template <typename T>
-void launch_physics(int kind, const Space<T> *space_dev, size_t npoints, hipStream_t st) {
-    const int nb = int((npoints + 255) / 256);
+void launch_physics(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return;
     switch (kind) {
-        case PHYS_ADV: k_adv_products<T><<<nb, 256, 0, st>>>(space_dev); break;
-        case PHYS_SGS_VISC: k_sgs_viscosity<T><<<nb, 256, 0, st>>>(space_dev); break;
-        case PHYS_SGS_STRESS: k_sgs_stress<T><<<nb, 256, 0, st>>>(space_dev); break;
+        case PHYS_ADV: k_adv_products<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
+        case PHYS_SGS_VISC: k_sgs_viscosity<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
+        case PHYS_SGS_STRESS: k_sgs_stress<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
     }
 }
 
 template <typename T>
-void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st) {
-    const size_t n = size_t(L.nk) * (3 * size_t(L.n3) - 1);
-    k_linear<T><<<int((n + 255) / 256), 256, 0, st>>>(L, sel);
+void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st) {
+    if (nblocks > 0) k_linear<T><<<nblocks, 256, 0, st>>>(descs_dev, lt, sel);
 }
 
+// One launch of the chunked stage kernel over all descriptors of the table;
+// CL is the chunk length shared by the launch (block = 256 threads = MB·NC).
 template <typename T>
-static bool launch_chunked(const ProjDesc<T> &D, int stage, double dt, int skip0, hipStream_t st) {
-    int CL = D.n3 <= 64 ? 4 : (D.n3 <= 128 ? 8 : 16);
-    const int NC = (D.n3 + CL - 1) / CL;
-    if (NC > 16) return false;
-    const int MB = D.nk == 1 ? 1 : std::max(4, 256 / NC);  // modes per block (block = MB * NC <= 256)
-    const int nb = (D.nk + MB - 1) / MB;
-    const size_t smem = size_t(NC) * MB * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<T>));
-    const int threads = MB * NC;
+void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int stage,
+                          double dt, hipStream_t st) {
+    if (nblocks <= 0) return;
+    const size_t smem = 256 * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<T>));
     if (CL == 4)
-        k_stage_chunked<T, 4><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
+        k_stage_chunked<T, 4><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
     else if (CL == 8)
-        k_stage_chunked<T, 8><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
+        k_stage_chunked<T, 8><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
     else
-        k_stage_chunked<T, 16><<<nb, threads, smem, st>>>(D, stage, dt, MB, skip0);
-    return true;
+        k_stage_chunked<T, 16><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
 }
 
+// one thread per column (very tall columns, n3 > 16·16)
 template <typename T>
-void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st,
-                          const ProjDesc<double> *D00) {
-    const bool mixed = D00 != nullptr;
-    if (launch_chunked<T>(D, stage, dt, mixed ? 1 : 0, st)) {
-        if (mixed) launch_chunked<double>(*D00, stage, dt, 0, st);  // float64 (0,0) mode
-        return;
-    }
+void launch_stage_sequential(const ProjDesc<T> &D, int stage, double dt, hipStream_t st) {
     k_stage_project<T><<<(D.nk + 63) / 64, 64, 0, st>>>(D, stage, dt, D.nk);
 }
 
@@ -1047,9 +1054,10 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
 
 #define BLD_INSTANTIATE(T)                                                                                         \
     template void launch_transform<T>(int, const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
-    template void launch_physics<T>(int, const Space<T> *, size_t, hipStream_t);                                  \
-    template void launch_linear<T>(const LinDesc<T> &, int, hipStream_t);                                         \
-    template void launch_stage_project<T>(const ProjDesc<T> &, int, double, hipStream_t, const ProjDesc<double> *); \
+    template void launch_physics<T>(int, const Space<T> *, const LaunchTable &, int, hipStream_t);                \
+    template void launch_linear<T>(const LinDesc<T> *, const LaunchTable &, int, int, hipStream_t);               \
+    template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t); \
+    template void launch_stage_sequential<T>(const ProjDesc<T> &, int, double, hipStream_t);                      \
     template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \
     template void launch_fill<T>(void *, size_t, hipStream_t);
 

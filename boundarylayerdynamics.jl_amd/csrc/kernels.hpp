@@ -41,6 +41,7 @@ struct ProjDesc {
     int *flag;
     // float64 (0,0)-mode pass of the mixed form: the float32 nonlinear rate of
     // the mode is added to du, and results are mirrored into the float32 state
+    int MB, skip0;                      // chunked kernel: modes per block; skip the (0,0) column
     int ext_stride;                     // 0: no extras
     const cplx<float> *du_add[3];
     cplx<float> *mirror_u[3], *mirror_ut[3];
@@ -50,12 +51,14 @@ template <typename T>
 void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel,
                       hipStream_t st);
 template <typename T>
-void launch_physics(int kind, const Space<T> *space_dev, size_t npoints, hipStream_t st);
+void launch_physics(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, hipStream_t st);
 template <typename T>
-void launch_linear(const LinDesc<T> &L, int sel, hipStream_t st);
-// D00: float64 pass over the (0,0) column of the mixed form (null in the plain form)
+void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st);
 template <typename T>
-void launch_stage_project(const ProjDesc<T> &D, int stage, double dt, hipStream_t st, const ProjDesc<double> *D00);
+void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int stage,
+                          double dt, hipStream_t st);
+template <typename T>
+void launch_stage_sequential(const ProjDesc<T> &D, int stage, double dt, hipStream_t st);
 template <typename T>
 void launch_check_finite(const void *a, size_t n, int *flag, hipStream_t st);
 template <typename T>

@@ -335,20 +335,30 @@ class Model:
 
     def __init__(self, resolution, domain, processes, precision="fp64"):
         disc = Discretization(resolution, domain, processes, precision)
-        self.disc = disc
-        self.k1max, self.k2max, self.n3 = disc.k1max, disc.k2max, disc.n3
-        self.domain, self.processes, self.precision, self.fields = domain, disc.processes, precision, disc.fields
         lib = _lib.load()
         h = ctypes.c_void_p()
         rc = lib.bld_create(ctypes.byref(disc.config), ctypes.byref(h))
         if rc != 0:
             raise BldError(rc, lib.bld_last_error(None).decode())
-        self._h = h
-        self._lib = lib
+        self._attach(disc, lib, h, 0, owner=True)
+
+    def _attach(self, disc, lib, h, index, owner):
+        self.disc = disc
+        self.k1max, self.k2max, self.n3 = disc.k1max, disc.k2max, disc.n3
+        self.domain, self.processes, self.precision, self.fields = (disc.domain, disc.processes,
+                                                                    disc.precision, disc.fields)
+        self._lib, self._h, self._inst, self._owner = lib, h, index, owner
+
+    @classmethod
+    def _member(cls, disc, lib, h, index, ensemble):
+        m = cls.__new__(cls)
+        m._attach(disc, lib, h, index, owner=False)
+        m._ensemble = ensemble  # keeps the engine alive
+        return m
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and getattr(self, "_owner", False):
             self._lib.bld_destroy(h)
             self._h = None
 
@@ -358,6 +368,9 @@ class Model:
 
     def nlayers(self, field):
         return self.n3 if NODES[field] == "C" else self.n3 - 1
+
+    def _use(self):
+        self._check(self._lib.bld_select_instance(self._h, self._inst))
 
     # -- engine calls ------------------------------------------------------
     def _check(self, rc):
@@ -374,6 +387,7 @@ class Model:
     def state(self):
         """Copy of the state (model.state of the reference) as host arrays."""
         out = {f: self._empty(f) for f in ("vel1", "vel2", "vel3")}
+        self._use()
         self._check(self._lib.bld_get_state(self._h, *self._triple(out)))
         return {f: out[f] for f in self.fields}
 
@@ -382,11 +396,13 @@ class Model:
         for f, a in arrays.items():
             if a.shape != (*self.fdsize, self.nlayers(f)):
                 raise ValueError(f"{f}: expected shape {(*self.fdsize, self.nlayers(f))}, got {a.shape}")
+        self._use()
         self._check(self._lib.bld_set_state(self._h, *self._triple(arrays)))
 
     def rates(self):
         """compute_rates! of the current state (Processes.jl:39-62)."""
         out = {f: self._empty(f) for f in ("vel1", "vel2", "vel3")}
+        self._use()
         self._check(self._lib.bld_rates(self._h, *self._triple(out)))
         return {f: out[f] for f in self.fields}
 
@@ -410,6 +426,7 @@ class Model:
         n1, n2 = self.default_size()
         if a.shape != (n1, n2, self.nlayers(field)):
             raise ValueError("physical values must have the default physical size")
+        self._use()
         self._check(self._lib.bld_set_field_physical(self._h, FIELD_INDEX[field], dptr(a)))
 
     def initialize(self, **ics):
@@ -431,6 +448,7 @@ class Model:
         """Default-size physical values (State.jl:81-85)."""
         n1, n2 = self.default_size()
         out = np.zeros((n1, n2, self.nlayers(field)), dtype=np.float64, order="F")
+        self._use()
         self._check(self._lib.bld_get_field_physical(self._h, FIELD_INDEX[field], dptr(out)))
         return out
 
@@ -448,6 +466,22 @@ class Model:
         ms = ctypes.c_double()
         self._check(self._lib.bld_time_steps(self._h, dt, nsteps, 1 if graph else 0, ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, reps=5):
+        """FFT batch autotuner (bld_autotune); recomputes the rates of the state."""
+        self._check(self._lib.bld_autotune(self._h, reps))
+
+    def tuning(self):
+        """{pass: [(batch, µs) per model space]} chosen by the autotuner."""
+        out = {}
+        for k, name in enumerate(("c2r_x2", "c2r_x1", "r2c_x1", "r2c_x2")):
+            out[name] = []
+            for e in range(16):
+                b, us = ctypes.c_int(), ctypes.c_double()
+                if self._lib.bld_tuning(self._h, k, e, ctypes.byref(b), ctypes.byref(us)) != 0:
+                    break
+                out[name].append((b.value, round(us.value, 2)))
+        return out
 
     def set_profiling(self, on):
         self._check(self._lib.bld_set_profiling(self._h, 1 if on else 0))
@@ -498,3 +532,52 @@ def evolve(model, tspan, dt, method=None, graph=True):
         nt = _approxdiv(t2 - t1, dt)
         return model.step(dt, nt, t=t1, t_end=t2, graph=graph)
     return t1
+
+
+class Ensemble:
+    """Independent models (uneven grids allowed) advanced together: one launch
+    per kernel class covers every member and a whole SSPRK33 step of the
+    ensemble is one HIP graph.  ``members[i]`` is a `Model` view of member i."""
+
+    def __init__(self, specs, precision="fp32"):
+        self.discs = [Discretization(res, dom, procs, precision) for res, dom, procs in specs]
+        lib = _lib.load()
+        cfgs = (BldConfig * len(self.discs))(*[d.config for d in self.discs])
+        h = ctypes.c_void_p()
+        rc = lib.bld_ensemble_create(len(self.discs), cfgs, ctypes.byref(h))
+        if rc != 0:
+            raise BldError(rc, lib.bld_last_error(None).decode())
+        self._lib, self._h = lib, h
+        self.members = [Model._member(d, lib, h, i, self) for i, d in enumerate(self.discs)]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.bld_destroy(h)
+            self._h = None
+
+    def __len__(self):
+        return len(self.members)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise BldError(rc, self._lib.bld_last_error(self._h).decode())
+
+    def init_problem(self, t1=0.0):
+        self._check(self._lib.bld_init_problem(self._h, t1))
+
+    def step(self, dt, nsteps, graph=True):
+        t = ctypes.c_double(0.0)
+        fn = self._lib.bld_step_graph if graph else self._lib.bld_step
+        self._check(fn(self._h, dt, nsteps, ctypes.byref(t), 0.0))
+
+    def time_steps(self, dt, nsteps, graph=True):
+        ms = ctypes.c_double()
+        self._check(self._lib.bld_time_steps(self._h, dt, nsteps, 1 if graph else 0, ctypes.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps=5):
+        self._check(self._lib.bld_autotune(self._h, reps))
+
+    def grid_points(self):
+        return sum(d.n3 * (2 * d.k1max + 2) * (2 * d.k2max + 2) for d in self.discs)

@@ -98,6 +98,28 @@ typedef struct bld_ldlt bld_ldlt;
 int bld_create(const bld_config *cfg, bld_engine **out);
 void bld_destroy(bld_engine *e);
 
+/* Ensemble of n independent models (uneven grids allowed, one precision):
+ * every step advances all of them, one launch per kernel class covering every
+ * model, the whole SSPRK33 step captured in one HIP graph.  No reference
+ * counterpart (the reference runs one Model per process). */
+int bld_ensemble_create(int n, const bld_config *cfgs, bld_engine **out);
+/* Number of models of an engine (1 for bld_create). */
+int bld_ensemble_size(bld_engine *e);
+/* Model that bld_set_state / bld_get_state / bld_rates (output) /
+ * bld_set_field_physical / bld_get_field_physical act on (default 0).
+ * bld_rates, bld_project, bld_init_problem and the step calls act on all. */
+int bld_select_instance(bld_engine *e, int index);
+
+/* FFT tile autotuner: times every power-of-two batch (columns / row pairs per
+ * block) of every transform pass and model space on the live state, keeps the
+ * fastest, and recomputes the rates of the current state (no reference
+ * counterpart; the reference relies on FFTW MEASURE plans,
+ * src/physical_space.jl:14). */
+int bld_autotune(bld_engine *e, int reps);
+/* Chosen batch and its time (µs) of transform pass (0 c2r_x2, 1 c2r_x1,
+ * 2 r2c_x1, 3 r2c_x2) for the entry-th model space. */
+int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us);
+
 /* Message of the last failed call on `e` (or of the last failed bld_create
  * on this thread when e is NULL). */
 const char *bld_last_error(const bld_engine *e);
