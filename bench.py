@@ -54,8 +54,10 @@ def make_les(B, precision, res):
     return m
 
 
-def time_form(B, precision, res, dt, steps, warmup, profile_steps):
+def time_form(B, precision, res, dt, steps, warmup, profile_steps, autotune=False):
     m = make_les(B, precision, res)
+    if autotune:
+        m.autotune(reps=3)
     m.step(dt, warmup, graph=True)
     t0 = time.perf_counter()
     ms_events = m.time_steps(dt, steps, graph=True)
@@ -164,6 +166,36 @@ def pressure_solve(B, nz=256, n=256, reps=5):
     return out
 
 
+def ensemble_bench(B, dt, steps, warmup, autotune):
+    """32 independent LES models of uneven grids (8 each of 48²×32, 64²×48,
+    96²×64, 128²×96) with seeded noise and Cs in [0.08, 0.16], fp32 fields,
+    advanced together; one HIP graph per ensemble step."""
+    W = B.workloads
+    rng = np.random.Generator(np.random.PCG64(17))
+    specs, grids = [], [(48, 48, 32), (64, 64, 48), (96, 96, 64), (128, 128, 96)]
+    for g in grids:
+        for _ in range(8):
+            specs.append(W.les_neutral(B, g, Cs=float(rng.uniform(0.08, 0.16))))
+    ens = B.Ensemble(specs, precision="fp32")
+    for i, m in enumerate(ens.members):
+        n1, n2 = m.default_size()
+        u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2, seed=1000 + i)
+        m.set_physical("vel1", u)
+        m.set_physical("vel2", v)
+        m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
+    ens.init_problem()
+    if autotune:
+        ens.autotune(reps=3)
+    ens.step(dt, warmup)
+    ms = ens.time_steps(dt, steps)
+    finite = all(np.all(np.isfinite(a)) for m in ens.members[::8] for a in m.state.values())
+    sps = 1e3 * steps / ms
+    pts = sum(g[0] * g[1] * g[2] * 8 for g in grids)
+    return dict(instances=len(ens), grid_points=pts, ensemble_steps_per_s=round(sps, 2),
+                ms_per_ensemble_step=round(ms / steps, 4), grid_point_updates_per_s=round(sps * pts, 1),
+                autotuned=autotune, finite=finite)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-pressure", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path "
@@ -184,7 +218,9 @@ def main():
     res = tuple(int(x) for x in args.res.split(","))
     B = load_package()
     log("fp32 mixed form ...")
-    m32, r32, kt32 = time_form(B, "fp32", res, args.dt, args.steps, args.warmup, args.profile_steps)
+    m32, r32, kt32 = time_form(B, "fp32", res, args.dt, args.steps, args.warmup, args.profile_steps,
+                               autotune=bool(args.autotune))
+    tuning32 = m32.tuning() if args.autotune else None
     adv_n = B.pdsize((m32.k1max, m32.k2max), "quadratic")
     sgs_n = B.pdsize((m32.k1max, m32.k2max), None)
     roof, per32 = roofline(B, m32, kt32, "fp32", res, adv_n, sgs_n)
@@ -193,7 +229,7 @@ def main():
     if not args.no_fp64:
         log("fp64 plain form ...")
         m64, r64, kt64 = time_form(B, "fp64", res, args.dt, max(args.steps // 3, 10), max(args.warmup // 3, 3),
-                                   args.profile_steps)
+                                   args.profile_steps, autotune=bool(args.autotune))
         roof64, per64 = roofline(B, m64, kt64, "fp64", res, adv_n, sgs_n)
         fp64 = dict(steps_per_s=round(1e3 / r64["ms_per_step"], 2), ms_per_step=round(r64["ms_per_step"], 4),
                     finite=r64["finite"], dominant_kernel=roof64["kernel"],
@@ -201,6 +237,11 @@ def main():
         del m64
         log("fp64:", r64)
     pres = None if args.no_pressure else pressure_solve(B)
+    ens = None
+    if not args.no_ensemble:
+        log("ensemble ...")
+        ens = ensemble_bench(B, args.dt, max(args.steps // 10, 10), 5, bool(args.autotune))
+        log("ensemble:", ens)
     cpu = None if args.no_cpu else cpu_baseline(res, args.dt, args.cpu_seconds)
     sps = 1e3 / r32["ms_per_step"]
     points = res[0] * res[1] * res[2]
@@ -221,6 +262,7 @@ def main():
                                f"free-slip top, static Smagorinsky Cs=0.1 with wall damping, SSPRK33 dt={args.dt}",
                    "form": "fp32 fields + fp64 pressure/mean mode (chosen form)",
                    "advection_grid": list(adv_n), "sgs_grid": list(sgs_n), "hip_graph": True,
+                   "fft_autotuned": bool(args.autotune),
                    "parallelism": "single GPU"},
         "grid_point_updates_per_s": round(sps * points, 1),
         "ms_per_step_hip_events": round(r32["ms_per_step_events"], 4),
@@ -230,7 +272,9 @@ def main():
         "kernels_fp32": {k: dict(avg_us=round(v["avg_us"], 2), launches=v["launches"], gbs=round(v["gbs"], 1))
                          for k, v in per32.items()},
         "roofline": roof,
+        "fft_autotune": tuning32,
         "pressure_solve_65536x256": pres,
+        "ensemble_32_uneven_les": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

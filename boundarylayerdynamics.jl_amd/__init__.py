@@ -13,3 +13,4 @@ from .model import (  # noqa: F401
 from . import _lib  # noqa: F401
 from .ldlt import BatchLDLt  # noqa: F401
 from . import workloads  # noqa: F401
+from . import closure  # noqa: F401

@@ -27,7 +27,9 @@ EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_siz
            "bld_autotune", "bld_tuning", "bld_last_error", "bld_set_state", "bld_get_state", "bld_rates",
            "bld_project", "bld_init_problem", "bld_step", "bld_step_graph", "bld_set_field_physical",
            "bld_get_field_physical", "bld_time_steps", "bld_set_profiling", "bld_kernel_times",
-           "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error")
+           "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error",
+           "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
+           "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time")
 
 
 class BldBC(ctypes.Structure):
@@ -67,6 +69,7 @@ def load():
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     D = ctypes.POINTER(ctypes.c_double)
+    F = ctypes.POINTER(ctypes.c_float)
     i64 = ctypes.c_int64
     sig = {
         "bld_create": ([ctypes.POINTER(BldConfig), ctypes.POINTER(P)], ctypes.c_int),
@@ -94,6 +97,16 @@ def load():
         "bld_ldlt_solve": ([P, D], ctypes.c_int),
         "bld_ldlt_bench": ([P, D, ctypes.c_int, D, D], ctypes.c_int),
         "bld_ldlt_last_error": ([P], ctypes.c_char_p),
+        "bld_mlp_nparams": ([], ctypes.c_int),
+        "bld_mlp_create": ([F, ctypes.POINTER(P)], ctypes.c_int),
+        "bld_mlp_destroy": ([P], None),
+        "bld_mlp_last_error": ([P], ctypes.c_char_p),
+        "bld_mlp_get_params": ([P, F], ctypes.c_int),
+        "bld_mlp_forward": ([P, F, i64, F], ctypes.c_int),
+        "bld_mlp_gradient": ([P, F, F, i64, F, D], ctypes.c_int),
+        "bld_mlp_set_data": ([P, F, F, i64], ctypes.c_int),
+        "bld_mlp_train": ([P, i64, i64, ctypes.c_float, D], ctypes.c_int),
+        "bld_mlp_time": ([P, i64, ctypes.c_int, D, D], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -115,3 +128,9 @@ def dptr(a):
         return ctypes.cast(None, ctypes.POINTER(ctypes.c_double))
     assert a.dtype in (np.float64, np.complex128) and (a.flags.f_contiguous or a.flags.c_contiguous)
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def fptr(a):
+    """float* of a contiguous float32 array."""
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))

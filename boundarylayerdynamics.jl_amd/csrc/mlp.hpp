@@ -1,0 +1,143 @@
+// Learned SGS closure network: 16 (10 used) → 64 → 64 → 16 (6 used), GELU,
+// bfloat16 operands on MFMA (v_mfma_f32_16x16x16_bf16) with float32 accumulation.
+//
+// MFMA 16x16x16 operand layout (lane l, r = l % 16, g = l / 16, element j = 0..3):
+//   A[m][k]: lane holds A[r][4g + j]     B[k][n]: lane holds B[4g + j][r]
+//   D[m][n]: lane holds D[4g + j][r]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bld {
+namespace mlp {
+
+constexpr int NIN = 10, NH = 64, NOUT = 6, PAD = 16;
+// parameter offsets (float32 master copy, row-major matrices [in][out])
+constexpr int OW1 = 0, OW2 = OW1 + PAD * NH, OW3 = OW2 + NH * NH, OB1 = OW3 + NH * PAD, OB2 = OB1 + NH,
+              OB3 = OB2 + NH, NPARAM = OB3 + PAD;
+// bfloat16 weight copy: W1, W2, W3 in the same row-major layout
+constexpr int NWB = OB1;
+
+typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
+typedef float floatx4 __attribute__((__vector_size__(4 * sizeof(float))));
+
+// round to nearest even, as bits
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float(uint32_t(b) << 16); }
+
+__device__ __forceinline__ float gelu(float z) {
+    const float s = 0.7978845608028654f;  // sqrt(2/π)
+    return 0.5f * z * (1.0f + tanhf(s * (z + 0.044715f * z * z * z)));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float s = 0.7978845608028654f;
+    const float t = tanhf(s * (z + 0.044715f * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * s * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+__device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// Forward pass of one 16-point tile by one wave.  `xa` is the A operand of the
+// input (lane holds x[point r][feature 4g+j] as bf16); weights in LDS (bf16,
+// row-major), biases in LDS (float).  `tile` is this wave's 16x64 bf16 LDS
+// scratch.  On return y holds D-layout outputs (lane: y[point 4g+j][out r]),
+// z1/z2 the D-layout pre-activations and h1d/h2d the D-layout bf16 activations.
+struct Fwd {
+    floatx4 z1[4], z2[4], y;
+    bf16x4 h1d[4], h2d[4];
+};
+
+__device__ __forceinline__ bf16x4 b_frag(const unsigned short *W, int ld, int k0, int n0, int lane) {
+    // B operand of rows k0.., columns n0..: lane holds W[k0 + 4g + j][n0 + r]
+    const int r = lane & 15, g = lane >> 4;
+    bf16x4 b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = short(W[(k0 + 4 * g + j) * ld + n0 + r]);
+    return b;
+}
+
+__device__ __forceinline__ bf16x4 bt_frag(const unsigned short *W, int ld, int k0, int n0, int lane) {
+    // B operand of the transpose: lane holds Wᵀ[k0 + 4g + j][n0 + r] = W[n0 + r][k0 + 4g + j]
+    const int r = lane & 15, g = lane >> 4;
+    bf16x4 b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = short(W[(n0 + r) * ld + k0 + 4 * g + j]);
+    return b;
+}
+
+// D-layout tile (16 x 16 block at column c0 of a 16-row LDS tile of width ld)
+// stored as bf16, and read back as the A operand (lane: T[r][k0 + 4g + j]).
+__device__ __forceinline__ void store_d(unsigned short *T, int ld, int c0, const floatx4 &v, int lane) {
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) T[(4 * g + j) * ld + c0 + r] = to_bf16(v[j]);
+}
+__device__ __forceinline__ bf16x4 a_frag(const unsigned short *T, int ld, int k0, int lane) {
+    const int r = lane & 15, g = lane >> 4;
+    bf16x4 a;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = short(T[r * ld + k0 + 4 * g + j]);
+    return a;
+}
+
+__device__ __forceinline__ void forward_tile(const bf16x4 &xa, const unsigned short *W, const float *bias,
+                                             unsigned short *tile, int lane, Fwd &o) {
+    const int r = lane & 15, g = lane >> 4;
+    const unsigned short *W1 = W + OW1, *W2 = W + OW2, *W3 = W + OW3;
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        floatx4 acc = mfma(xa, b_frag(W1, NH, 0, 16 * t, lane), zero);
+        floatx4 h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[j] += bias[OB1 - OB1 + 16 * t + r];
+            h[j] = gelu(acc[j]);
+            o.h1d[t][j] = short(to_bf16(h[j]));
+        }
+        o.z1[t] = acc;
+        store_d(tile, NH, 16 * t, h, lane);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+    bf16x4 a[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = a_frag(tile, NH, 16 * c, lane);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        floatx4 acc = zero;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc = mfma(a[c], b_frag(W2, NH, 16 * c, 16 * t, lane), acc);
+        floatx4 h;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[j] += bias[NH + 16 * t + r];
+            h[j] = gelu(acc[j]);
+            o.h2d[t][j] = short(to_bf16(h[j]));
+        }
+        o.z2[t] = acc;
+        store_d(tile, NH, 16 * t, h, lane);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a[c] = a_frag(tile, NH, 16 * c, lane);
+    __builtin_amdgcn_wave_barrier();
+    floatx4 acc = zero;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc = mfma(a[c], b_frag(W3, PAD, 16 * c, 0, lane), acc);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] += bias[2 * NH + r];
+    o.y = acc;
+    (void)g;
+}
+
+}  // namespace mlp
+}  // namespace bld

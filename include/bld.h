@@ -179,6 +179,28 @@ int bld_ldlt_solve(bld_ldlt *s, double *b);
 int bld_ldlt_bench(bld_ldlt *s, const double *b, int reps, double *ms_per_solve, double *x_out);
 const char *bld_ldlt_last_error(const bld_ldlt *s);
 
+/* Learned SGS closure network (no reference counterpart; it sits behind the
+ * process API next to StaticSmagorinskyModel, static_smagorinsky.jl:39-42):
+ * 10 features (padded to 16) → 64 → 64 (GELU) → 6 outputs (padded to 16),
+ * bfloat16 MFMA with float32 accumulation.  Parameters are float32, row-major
+ * [in][out] matrices W1[16][64], W2[64][64], W3[64][16], then b1[64], b2[64],
+ * b3[16] (bld_mlp_nparams() floats).  Inputs/outputs are float32 [n][16]. */
+typedef struct bld_mlp bld_mlp;
+int bld_mlp_nparams(void);
+int bld_mlp_create(const float *params, bld_mlp **out);
+void bld_mlp_destroy(bld_mlp *h);
+const char *bld_mlp_last_error(const bld_mlp *h);
+int bld_mlp_get_params(bld_mlp *h, float *params);
+int bld_mlp_forward(bld_mlp *h, const float *x, int64_t n, float *y);
+/* Gradient of the mean squared error over the 6 outputs of (x, t). */
+int bld_mlp_gradient(bld_mlp *h, const float *x, const float *t, int64_t n, float *grad, double *loss);
+/* Training set (device resident) and `steps` Adam steps over consecutive
+ * mini-batches of `batch` points; losses[steps] receives the batch losses. */
+int bld_mlp_set_data(bld_mlp *h, const float *x, const float *t, int64_t n);
+int bld_mlp_train(bld_mlp *h, int64_t steps, int64_t batch, float lr, double *losses);
+/* Average milliseconds of one forward pass and one gradient pass over n points. */
+int bld_mlp_time(bld_mlp *h, int64_t n, int reps, double *ms_forward, double *ms_gradient);
+
 #ifdef __cplusplus
 }
 #endif

@@ -300,3 +300,80 @@ def test_golden_vectors(name):
         got = gpu.state
         for f in gpu.fields:
             assert relerr(got[f], g[f"step_{f}"]) < RTOL64
+
+
+# -- ensembles: several models of uneven size advanced by the same launches ----------
+def test_ensemble_matches_oracle():
+    names = ["channel", "les", "stretched", "tiny"]
+    refs, specs, states = [], [], []
+    for i, name in enumerate(names):
+        ref = make(O, name)
+        st = random_state(ref, np.random.default_rng(40 + i))
+        for f in ref.fields:
+            ref.state[f][...] = st[f]
+        refs.append(ref)
+        states.append(st)
+        c = CASES[name]
+        mapping = B.SinusoidalMapping(c["mapping"]) if "mapping" in c else None
+        dom = B.Domain(c["dims"], _boundary(B, c["lower"]), _boundary(B, c["upper"]), mapping)
+        specs.append((c["res"], dom, c["procs"](B)))
+    ens = B.Ensemble(specs, precision="fp64")
+    assert len(ens) == len(names)
+    for m, st in zip(ens.members, states):
+        m.set_state(**st)
+    dt, steps = 1e-3, 3
+    ens.init_problem()
+    ens.step(dt, steps)
+    for ref, m, name in zip(refs, ens.members, names):
+        O.evolve(ref, steps * dt, dt)
+        got = m.state
+        for f in ref.fields:
+            assert relerr(got[f], ref.state[f]) < RTOL64, (name, f, relerr(got[f], ref.state[f]))
+
+
+def test_ensemble_fp32_matches_single_models():
+    """Uneven LES grids in one fp32 ensemble equal the same models run alone."""
+    res_list = [(12, 12, 6), (16, 16, 8), (20, 20, 10)]
+    W = B.workloads
+    specs, singles, states = [], [], []
+    for i, res in enumerate(res_list):
+        n, dom, procs = W.les_neutral(B, res, Cs=0.08 + 0.03 * i)
+        specs.append((n, dom, procs))
+        single = B.Model(n, dom, procs, precision="fp32")
+        ref = make(O, "les")  # only for shapes
+        n1, n2 = single.default_size()
+        u, v = W.les_initial(single.coordinates("vel1", 3), n1, n2, seed=50 + i)
+        single.set_physical("vel1", u)
+        single.set_physical("vel2", v)
+        single.set_physical("vel3", np.zeros((n1, n2, single.n3 - 1), order="F"))
+        states.append(single.state)
+        singles.append(single)
+    ens = B.Ensemble(specs, precision="fp32")
+    for m, st in zip(ens.members, states):
+        m.set_state(**st)
+    ens.init_problem()
+    ens.step(5e-4, 4)
+    for s in singles:
+        s.init_problem()
+        s.step(5e-4, 4)
+    for m, s in zip(ens.members, singles):
+        a, b = m.state, s.state
+        for f in m.fields:
+            assert relerr(a[f], b[f]) < 1e-5, (f, relerr(a[f], b[f]))
+
+
+@pytest.mark.parametrize("name", ["les", "channel"])
+def test_autotune_keeps_results(name):
+    ref, gpu = pair(name, seed=60)
+    before = gpu.rates()
+    gpu.autotune(reps=2)
+    tun = gpu.tuning()
+    assert all(len(v) >= 1 for v in tun.values())
+    after = gpu.rates()
+    for f in ref.fields:
+        assert relerr(after[f], before[f]) < 1e-14
+    O.evolve(ref, 3e-3, 1e-3)
+    B.evolve(gpu, 3e-3, 1e-3)
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < RTOL64
