@@ -175,7 +175,7 @@ def ensemble_bench(B, dt, steps, warmup, autotune):
     specs, grids = [], [(48, 48, 32), (64, 64, 48), (96, 96, 64), (128, 128, 96)]
     for g in grids:
         for _ in range(8):
-            specs.append(W.les_neutral(B, g, Cs=float(rng.uniform(0.08, 0.16))))
+            specs.append(W.les_neutral(B, g, Cs=float(rng.uniform(0.08, 0.16)), fft_friendly=True))
     ens = B.Ensemble(specs, precision="fp32")
     for i, m in enumerate(ens.members):
         n1, n2 = m.default_size()
@@ -188,12 +188,17 @@ def ensemble_bench(B, dt, steps, warmup, autotune):
         ens.autotune(reps=3)
     ens.step(dt, warmup)
     ms = ens.time_steps(dt, steps)
+    ens.set_profiling(True)
+    ens.time_steps(dt, 5, graph=False)
+    kt = {k: round(v[0] / 5, 4) for k, v in ens.kernel_times().items() if v[1]}
+    ens.set_profiling(False)
     finite = all(np.all(np.isfinite(a)) for m in ens.members[::8] for a in m.state.values())
     sps = 1e3 * steps / ms
     pts = sum(g[0] * g[1] * g[2] * 8 for g in grids)
-    return dict(instances=len(ens), grid_points=pts, ensemble_steps_per_s=round(sps, 2),
+    adv = sorted({tuple(s[2][0].dealiasing) for s in specs})
+    return dict(instances=len(ens), grid_points=pts, advection_grids=[list(a) for a in adv], ensemble_steps_per_s=round(sps, 2),
                 ms_per_ensemble_step=round(ms / steps, 4), grid_point_updates_per_s=round(sps * pts, 1),
-                autotuned=autotune, finite=finite)
+                autotuned=autotune, finite=finite, kernel_ms_per_step=kt)
 
 
 def main():

@@ -20,7 +20,7 @@ BLD_WALL_FREESLIP, BLD_WALL_ROUGH = 0, 1
 
 # kernel classes reported by bld_kernel_times (order of the engine's KernelKind)
 KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "sgs_stress", "r2c_x1",
-                "r2c_x2", "stage_project", "check_finite")
+                "r2c_x2", "stage_project", "check_finite", "sgs_mlp")
 
 # every symbol include/bld.h declares
 EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_size", "bld_select_instance",
@@ -29,7 +29,8 @@ EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_siz
            "bld_get_field_physical", "bld_time_steps", "bld_set_profiling", "bld_kernel_times",
            "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error",
            "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
-           "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time")
+           "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time",
+           "bld_sgs_closure", "bld_closure_sample")
 
 
 class BldBC(ctypes.Structure):
@@ -107,6 +108,8 @@ def load():
         "bld_mlp_set_data": ([P, F, F, i64], ctypes.c_int),
         "bld_mlp_train": ([P, i64, i64, ctypes.c_float, D], ctypes.c_int),
         "bld_mlp_time": ([P, i64, ctypes.c_int, D, D], ctypes.c_int),
+        "bld_sgs_closure": ([P, F, ctypes.c_double, D, ctypes.c_int], ctypes.c_int),
+        "bld_closure_sample": ([P, ctypes.c_int, ctypes.c_double, D, F, F, i64, ctypes.POINTER(i64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

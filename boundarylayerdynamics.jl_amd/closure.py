@@ -101,3 +101,85 @@ class MLP:
         f, g = ctypes.c_double(), ctypes.c_double()
         self._check(self._lib.bld_mlp_time(self._h, n, reps, ctypes.byref(f), ctypes.byref(g)))
         return f.value, g.value
+
+
+# ---------------------------------------------------------------------------
+# training on filtered-DNS samples and the a-priori comparison with Smagorinsky
+# ---------------------------------------------------------------------------
+
+
+def fold_output_scale(params, scale):
+    """Parameters whose outputs are `scale` times those of `params` (W3, b3 columns)."""
+    p = {k: np.array(v, dtype=np.float64) for k, v in params.items()}
+    s = np.zeros(PAD)
+    s[:NOUT] = scale
+    p["W3"] = p["W3"] * s[None, :]
+    p["b3"] = p["b3"] * s
+    return p
+
+
+def train(x, t, steps=2000, batch=8192, lr=2e-3, seed=0, chunks=4):
+    """Trains the closure network on the GPU (Adam over consecutive mini-batches
+    of the device-resident samples) on targets standardised per component; the
+    standardisation is folded into the output layer.  `chunks` splits `steps`
+    into runs with a decayed learning rate (lr, lr/3, lr/9, ...).  Returns
+    (params dict, losses of the standardised problem)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    order = rng.permutation(x.shape[0])
+    x, t = np.asarray(x, dtype=np.float32)[order], np.asarray(t, dtype=np.float32)[order]
+    scale = t[:, :NOUT].std(axis=0).astype(np.float64)
+    scale[scale == 0] = 1.0
+    ts = np.zeros_like(t)
+    ts[:, :NOUT] = t[:, :NOUT] / scale[None, :].astype(np.float32)
+    net = MLP(seed=seed)
+    net.set_data(x, ts)
+    losses = []
+    per = max(1, steps // chunks)
+    for c in range(chunks):
+        losses.append(net.train(per, batch=min(batch, x.shape[0]), lr=lr / 3 ** c))
+    return fold_output_scale(net.params(), scale), np.concatenate(losses)
+
+
+def smagorinsky_from_features(x, Cs=0.1, kappa=0.4, damping_exponent=2, wall_damping=True):
+    """Static Smagorinsky (static_smagorinsky.jl:125-127, with Mason wall
+    damping towards the wall the wall feature measures) written in the
+    closure's normalised variables: y = τ^dev/(Δ²|g|²) = −2 (l/Δ)² |Ŝ| Ŝ with
+    Ŝ = sym(g/|g|) and l/Δ = (Cs^-N + (κ d/Δ)^-N)^(-1/N), d/Δ = 1/x10 − 1
+    (l/Δ = Cs without damping)."""
+    g = np.asarray(x[:, :9], dtype=np.float64).reshape(-1, 3, 3)
+    S = 0.5 * (g + g.transpose(0, 2, 1))
+    mag = np.sqrt(2 * np.einsum("nij,nij->n", S, S))
+    xw = np.asarray(x[:, 9], dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        d = np.where(xw > 0, 1.0 / np.maximum(xw, 1e-300) - 1.0, np.inf)
+        N = damping_exponent
+        l = (Cs ** (-N) + (kappa * d) ** (-N)) ** (-1.0 / N) if wall_damping else np.full_like(d, Cs)
+    tau = -2 * (l ** 2 * mag)[:, None, None] * S
+    tr = np.trace(tau, axis1=1, axis2=2) / 3
+    tau = tau - tr[:, None, None] * np.eye(3)[None]
+    return np.stack([tau[:, 0, 0], tau[:, 1, 1], tau[:, 2, 2], tau[:, 0, 1], tau[:, 0, 2], tau[:, 1, 2]], axis=1)
+
+
+def apriori(params, x, t, Cs=0.1):
+    """Error of the learned stress, of Smagorinsky with constant `Cs` and of
+    undamped Smagorinsky with its best constant (least squares over these samples),
+    against the filtered-DNS stress, in dimensional units (t[:, 15] = Δ²|ḡ|²):
+    relative squared error Σ|τ − τ_true|²/Σ|τ_true|² and the correlation
+    coefficient of each component."""
+    scale = np.asarray(t[:, 15], dtype=np.float64)[:, None]
+    true = np.asarray(t[:, :NOUT], dtype=np.float64) * scale
+    w = np.array([1, 1, 1, 2, 2, 2], dtype=np.float64)  # off-diagonal entries count twice in |τ|²
+
+    def metrics(pred):
+        err = float(np.sum(w * (pred - true) ** 2) / np.sum(w * true ** 2))
+        corr = [float(np.corrcoef(pred[:, m], true[:, m])[0, 1]) if np.std(pred[:, m]) > 0 else 0.0
+                for m in range(NOUT)]
+        return dict(rel_sq_error=err, corr=corr)
+
+    learned = MLP(params).forward(x).astype(np.float64) * scale
+    smag = smagorinsky_from_features(x, Cs) * scale
+    m = smagorinsky_from_features(x, 1.0, wall_damping=False) * scale
+    c2 = float(np.sum(w * m * true) / np.sum(w * m * m))
+    best = m * c2
+    return dict(learned=metrics(learned), smagorinsky=metrics(smag), smagorinsky_best=metrics(best),
+                best_cs2=c2)

@@ -101,6 +101,13 @@ struct Space {
     int wall_rough[2];
     double fstrain[2], fstress[2];
     double wbc[2];                    // vel3 Dirichlet values (physical)
+    // learned closure (SGS spaces whose stresses come from the network)
+    const unsigned short *mlp_w;      // bfloat16 W1, W2, W3
+    const float *mlp_b;               // b1, b2, b3
+    const double *mlp_xw;             // wall feature per C-layer [n3]
+    double mlp_delta2;                // Δ² of the stress scale Δ²|∇u|²
+    int mlp_guard;                    // Smagorinsky where the learned stress is not dissipative
+    T *tau[6];                        // C-node stresses τ11, τ22, τ33, τ12, τ13, τ23
 };
 
 // Per-launch block bookkeeping of a multi-space transform launch.
@@ -118,5 +125,18 @@ struct LaunchTable {
     int n;
     const LaunchEntry *e;
 };
+
+// entry of block b (entries sorted by block_begin)
+__device__ __forceinline__ int find_entry(const LaunchTable &lt, int b) {
+    int lo = 0, hi = lt.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (b >= lt.e[mid].block_begin)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 }  // namespace bld

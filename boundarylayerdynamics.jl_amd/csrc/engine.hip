@@ -11,6 +11,7 @@
 
 #include "bld.h"
 #include "kernels.hpp"
+#include "mlp.hpp"
 
 namespace bld {
 
@@ -94,6 +95,20 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
@@ -135,11 +150,14 @@ struct EngineBase {
     virtual void kernel_time(int kind, double *ms, int64_t *count) = 0;
     virtual void autotune(int reps) = 0;
     virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
+    virtual void set_closure(const float *params, double delta, const double *xw, int flags) = 0;
+    virtual void sample_closure(int F, double delta, const double *xw, float *X, float *Y, int64_t cap,
+                                int64_t *count) = 0;
 };
 
 enum KernelKind {
     KK_LINEAR = 0, KK_C2R_X2, KK_C2R_X1, KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_R2C_X1, KK_R2C_X2,
-    KK_STAGE, KK_CHECK, KK_COUNT
+    KK_STAGE, KK_CHECK, KK_SGS_MLP, KK_COUNT
 };
 
 // One model (one bld_config): its device state, constants and descriptors.
@@ -153,6 +171,9 @@ struct Instance {
     DevBuf d_D1, d_D2, d_DD1, d_DD2, d_ac, d_ai, d_lc, d_li, d_gamma, d_gamma0, d_pbuf, d_xfer_phys;
     std::vector<std::unique_ptr<DevBuf>> owned;
     int adv = -1, sgs = -1, xfer = -1;  // indices into the engine's space array
+    int ana = -1;                        // analysis space of the filtered-DNS sampler (created on demand)
+    bool closure = false;                // SGS stresses from the learned closure
+    DevBuf d_mlp_w, d_mlp_b, d_mlp_xw;
     LinDesc<T> lin{};
     ProjDesc<T> proj{};
     ProjDesc<double> proj00{};
@@ -243,9 +264,13 @@ class Engine : public EngineBase {
         size_t smem = 0;
         LaunchTable lt() const { return LaunchTable{int(ent.size()), dev.template as<LaunchEntry>()}; }
     };
-    Pass tp[4];          // transform passes over all spaces (r2c_x2 only advection spaces)
-    Pass tp_r2c_x2_sgs;  // r2c_x2 of the SGS spaces (accumulates into the same rates)
-    Pass pp[3];          // physics: advection products, SGS viscosity, SGS stress
+    // transform passes over all model spaces, one launch per LDS class so that
+    // small transforms do not run at the LDS footprint of the largest one;
+    // r2c_x2 of the SGS spaces is separate (it accumulates into the same rates)
+    std::vector<Pass> tp[4];
+    std::vector<Pass> tp_r2c_x2_sgs;
+    std::vector<std::pair<int, int>> tp_entry_space[5];  // (space, batch) per pass, for bld_tuning
+    Pass pp[5];  // physics: advection products, SGS viscosity, SGS stress, learned stress, its rates
     Pass lin_pass, stage_pass, stage00_pass, xfer[4];
     int stage_CL = 4;
     // tuning results: per transform pass and entry
@@ -621,10 +646,28 @@ class Engine : public EngineBase {
         ps.smem = 0;
     }
 
+    static int lds_class(size_t lds) { return lds <= 16 * 1024 ? 0 : (lds <= 32 * 1024 ? 1 : 2); }
+
+    void bucket_add(std::vector<Pass> &v, int kind, int sidx, int batch) {
+        if (v.empty()) v.resize(3);
+        Pass probe;
+        add_entry(probe, kind, sidx, batch);
+        add_entry(v[lds_class(probe.smem)], kind, sidx, batch);
+    }
+
+    void bucket_finish(std::vector<Pass> &v) {
+        std::vector<Pass> keep;
+        for (auto &p : v)
+            if (!p.ent.empty()) keep.push_back(std::move(p));
+        v = std::move(keep);
+        for (auto &p : v) finish(p);
+    }
+
     // batches[k][i]: batch of transform pass k for the i-th model space (0 = default)
     void build_transform_passes(const std::vector<std::vector<int>> *batches = nullptr) {
-        for (int k = 0; k < 4; ++k) reset(tp[k]);
-        reset(tp_r2c_x2_sgs);
+        for (int k = 0; k < 4; ++k) tp[k].clear();
+        tp_r2c_x2_sgs.clear();
+        for (int k = 0; k < 5; ++k) tp_entry_space[k].clear();
         std::vector<int> model_spaces;
         for (auto &I : inst)
             for (int s : {I->adv, I->sgs})
@@ -632,36 +675,48 @@ class Engine : public EngineBase {
         for (size_t i = 0; i < model_spaces.size(); ++i) {
             const int s = model_spaces[i];
             auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
-            for (int k = 0; k < 3; ++k) add_entry(tp[k], k, s, b(k));
+            for (int k = 0; k < 3; ++k) bucket_add(tp[k], k, s, b(k));
             if (spaces[s].kind == SK_ADVECTION)
-                add_entry(tp[PASS_R2C_X2], PASS_R2C_X2, s, b(3));
+                bucket_add(tp[PASS_R2C_X2], PASS_R2C_X2, s, b(3));
             else
-                add_entry(tp_r2c_x2_sgs, PASS_R2C_X2, s, b(3));
+                bucket_add(tp_r2c_x2_sgs, PASS_R2C_X2, s, b(3));
+            for (int k = 0; k < 4; ++k) {
+                int n, bufs, avail, njobs;
+                pass_geometry(k, spaces[s], n, bufs, avail, njobs);
+                tp_entry_space[k].push_back({s, b(k) > 0 ? b(k) : default_batch(n, bufs, avail)});
+            }
         }
-        for (int k = 0; k < 4; ++k) finish(tp[k]);
-        finish(tp_r2c_x2_sgs);
+        for (int k = 0; k < 4; ++k) bucket_finish(tp[k]);
+        bucket_finish(tp_r2c_x2_sgs);
     }
 
     void build_plans() {
         build_transform_passes();
-        // physics and linear terms: 256 threads per block, one entry per space / model
-        for (int k = 0; k < 3; ++k) reset(pp[k]);
+        build_point_plans();
+    }
+
+    // physics, linear terms and stages: 256 threads per block, one entry per space / model
+    void build_point_plans() {
+        for (int k = 0; k < 5; ++k) reset(pp[k]);
         reset(lin_pass);
         for (size_t i = 0; i < inst.size(); ++i) {
             Instance<T> &I = *inst[i];
-            auto add_points = [&](Pass &ps, int sidx) {
+            auto add_points = [&](Pass &ps, int sidx, int per_block = 256) {
                 const Space<T> &S = spaces[sidx];
                 LaunchEntry E{};
                 E.space = sidx;
                 E.block_begin = ps.nblocks;
                 const size_t pts = size_t(S.N1) * S.N2 * S.n3;
-                E.blocks_per_job = int((pts + 255) / 256);
+                E.blocks_per_job = int((pts + per_block - 1) / per_block);
                 E.njobs = 1;
                 ps.ent.push_back(E);
                 ps.nblocks += E.blocks_per_job;
             };
             if (I.adv >= 0) add_points(pp[PHYS_ADV], I.adv);
-            if (I.sgs >= 0) {
+            if (I.sgs >= 0 && I.closure) {
+                add_points(pp[PHYS_SGS_MLP], I.sgs, 256 * kMlpChunks);
+                add_points(pp[PHYS_SGS_MLP_STRESS], I.sgs);
+            } else if (I.sgs >= 0) {
                 add_points(pp[PHYS_SGS_VISC], I.sgs);
                 add_points(pp[PHYS_SGS_STRESS], I.sgs);
             }
@@ -673,7 +728,7 @@ class Engine : public EngineBase {
             lin_pass.ent.push_back(E);
             lin_pass.nblocks += E.blocks_per_job;
         }
-        for (int k = 0; k < 3; ++k) finish(pp[k]);
+        for (int k = 0; k < 5; ++k) finish(pp[k]);
         finish(lin_pass);
         std::vector<LinDesc<T>> lins;
         for (auto &I : inst) lins.push_back(I->lin);
@@ -756,16 +811,26 @@ class Engine : public EngineBase {
         timed(kk, [&] { launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt(), p.nblocks, p.smem, sel, st); });
     }
 
+    void run_transform(int kind, const std::vector<Pass> &v, int sel, int kk) {
+        if (v.empty()) return;
+        timed(kk, [&] {
+            for (const Pass &p : v)
+                if (p.nblocks) launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt(), p.nblocks, p.smem, sel, st);
+        });
+    }
+
     // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du
     void rhs(int sel) {
         timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
-        if (tp[PASS_C2R_X2].nblocks == 0) return;
+        if (tp[PASS_C2R_X2].empty()) return;
         run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
         run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
         const Space<T> *dsp = d_spaces.as<Space<T>>();
-        const int kks[3] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS};
+        const int kks[5] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_SGS_MLP, KK_SGS_STRESS};
         for (int k = 0; k < 3; ++k)
             if (pp[k].nblocks) timed(kks[k], [&] { launch_physics<T>(k, dsp, pp[k].lt(), pp[k].nblocks, st); });
+        for (int k = 3; k < 5; ++k)
+            if (pp[k].nblocks) timed(kks[k], [&] { launch_closure<T>(k, dsp, pp[k].lt(), pp[k].nblocks, st); });
         run_transform(PASS_R2C_X1, tp[PASS_R2C_X1], sel, KK_R2C_X1);
         run_transform(PASS_R2C_X2, tp[PASS_R2C_X2], sel, KK_R2C_X2);
         run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
@@ -985,7 +1050,7 @@ class Engine : public EngineBase {
                 pass_geometry(k, spaces[s], n, bufs, avail, njobs);
                 double bt = 1e30;
                 for (int b = 1; b <= 32 && b <= std::max(1, avail); b *= 2) {
-                    if (size_t(bufs) * b * n * sizeof(cplx<T>) > 96 * 1024) break;
+                    if (size_t(bufs) * b * n * sizeof(cplx<T>) > 64 * 1024) break;
                     const double us = time_entry(k, s, b, 0, reps);
                     if (us < bt) {
                         bt = us;
@@ -1006,15 +1071,10 @@ class Engine : public EngineBase {
 
     void tuning(int pass, int entry, int *batch, double *us) override {
         if (pass < 0 || pass > 3) throw BldError{BLD_EARG, "unknown pass"};
-        const Pass &p = pass == 3 ? tp[3] : tp[pass];
-        // entries of pass 3 are split between the advection and SGS launches
-        std::vector<const LaunchEntry *> all;
-        for (auto &e : p.ent) all.push_back(&e);
-        if (pass == 3)
-            for (auto &e : tp_r2c_x2_sgs.ent) all.push_back(&e);
-        if (entry < 0 || entry >= int(all.size())) throw BldError{BLD_EARG, "entry index out of range"};
-        *batch = all[entry]->batch;
-        *us = (pass < 4 && entry < int(tuned_us[pass].size())) ? tuned_us[pass][entry] : 0.0;
+        const auto &v = tp_entry_space[pass];
+        if (entry < 0 || entry >= int(v.size())) throw BldError{BLD_EARG, "entry index out of range"};
+        *batch = v[entry].second;
+        *us = entry < int(tuned_us[pass].size()) ? tuned_us[pass][entry] : 0.0;
     }
 
     // -- physical-space transfer at the default size (selected model) -------------
@@ -1060,6 +1120,94 @@ class Engine : public EngineBase {
             }
         }
         HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // -- learned closure ------------------------------------------------------------
+    static unsigned short bf16_bits(float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    }
+
+    void drop_graph() {
+        if (graph_exec) {
+            HIPCHECK(hipGraphExecDestroy(graph_exec));
+            graph_exec = nullptr;
+        }
+    }
+
+    void set_closure(const float *params, double delta, const double *xw, int flags) override {
+        Instance<T> &I = cur();
+        if (I.sgs < 0)
+            throw BldError{BLD_EUNSUPPORTED, "the learned closure needs the model's SGS process, which is its fallback"};
+        Space<T> &S = spaces[I.sgs];
+        if (!params) {
+            I.closure = false;
+        } else {
+            if (!xw || !(delta > 0)) throw BldError{BLD_EARG, "learned closure: missing wall feature or Δ <= 0"};
+            for (int i = 0; i < mlp::NPARAM; ++i)
+                if (!std::isfinite(params[i])) throw BldError{BLD_EARG, "learned closure: non-finite parameter"};
+            std::vector<unsigned short> w(mlp::NWB);
+            for (int i = 0; i < mlp::NWB; ++i) w[i] = bf16_bits(params[i]);
+            upload(I.d_mlp_w, w);
+            upload(I.d_mlp_b, std::vector<float>(params + mlp::OB1, params + mlp::NPARAM));
+            upload(I.d_mlp_xw, std::vector<double>(xw, xw + I.n3));
+            const size_t n = size_t(S.N1) * S.N2 * I.n3;
+            if (!S.tau[0])
+                for (int m = 0; m < 6; ++m) S.tau[m] = I.own(n * sizeof(T)).template as<T>();
+            S.mlp_w = I.d_mlp_w.template as<unsigned short>();
+            S.mlp_b = I.d_mlp_b.template as<float>();
+            S.mlp_xw = I.d_mlp_xw.template as<double>();
+            S.mlp_delta2 = delta * delta;
+            S.mlp_guard = (flags & BLD_CLOSURE_GUARD) ? 1 : 0;
+            I.closure = true;
+        }
+        sync_spaces();
+        build_point_plans();
+        drop_graph();
+        HIPCHECK(hipDeviceSynchronize());
+    }
+
+    void sample_closure(int F, double delta, const double *xw, float *X, float *Y, int64_t cap,
+                        int64_t *count) override {
+        Instance<T> &I = cur();
+        const int N1 = 2 * I.cfg.k1max + 2, N2 = 2 * I.cfg.k2max + 2;
+        if (F < 1 || N1 % F || N2 % F) throw BldError{BLD_EARG, "the filter width must divide the physical grid"};
+        if (!xw || !(delta > 0)) throw BldError{BLD_EARG, "sampler: missing wall feature or Δ <= 0"};
+        if (I.cfg.bc[2][0].kind != BLD_BC_DIRICHLET || I.cfg.bc[2][1].kind != BLD_BC_DIRICHLET)
+            throw BldError{BLD_EUNSUPPORTED, "the sampler requires Dirichlet boundary conditions for vel3"};
+        const int64_t n = int64_t(N1 / F) * (N2 / F) * I.n3;
+        *count = n;
+        if (!X || !Y) return;  // size query
+        if (cap < n) throw BldError{BLD_EARG, "sample buffers too small"};
+        if (I.ana < 0) {
+            I.ana = add_space(I, SK_SGS, N1, N2);
+            spaces[I.ana].tw = d_tw.as<cplx<T>>();
+            DevBuf grown;
+            grown.alloc(spaces.size() * sizeof(Space<T>));
+            d_spaces = std::move(grown);
+            upload(d_tw, tw.host);
+            for (auto &sp : spaces) sp.tw = d_tw.template as<cplx<T>>();
+            sync_spaces();
+            drop_graph();
+        }
+        Pass a2, a1;
+        add_entry(a2, PASS_C2R_X2, I.ana);
+        add_entry(a1, PASS_C2R_X1, I.ana);
+        finish(a2);
+        finish(a1);
+        run_transform(PASS_C2R_X2, a2, 0, KK_C2R_X2);
+        run_transform(PASS_C2R_X1, a1, 0, KK_C2R_X1);
+        DevBuf dxw, dX, dY;
+        upload(dxw, std::vector<double>(xw, xw + I.n3));
+        dX.alloc(size_t(n) * mlp::PAD * sizeof(float));
+        dY.alloc(size_t(n) * mlp::PAD * sizeof(float));
+        launch_closure_sample<T>(d_spaces.as<Space<T>>() + I.ana, N1, N2, I.n3, F, delta * delta,
+                                 dxw.as<double>(), dX.as<float>(), dY.as<float>(), st);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipMemcpy(X, dX.p, size_t(n) * mlp::PAD * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(Y, dY.p, size_t(n) * mlp::PAD * sizeof(float), hipMemcpyDeviceToHost));
     }
 
     void get_field_physical(int f, double *v) override {
@@ -1259,6 +1407,22 @@ int bld_ensemble_size(bld_engine *e) {
 int bld_select_instance(bld_engine *e, int index) {
     ENGINE_CALL({
         e->impl->select(index);
+        return BLD_OK;
+    })
+}
+
+int bld_sgs_closure(bld_engine *e, const float *params, double delta, const double *wall_feature, int flags) {
+    ENGINE_CALL({
+        e->impl->set_closure(params, delta, wall_feature, flags);
+        return BLD_OK;
+    })
+}
+
+int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wall_feature, float *x, float *t,
+                       int64_t capacity, int64_t *count) {
+    if (!count) return BLD_EARG;
+    ENGINE_CALL({
+        e->impl->sample_closure(filter, delta, wall_feature, x, t, capacity, count);
         return BLD_OK;
     })
 }

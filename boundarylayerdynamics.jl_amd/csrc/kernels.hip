@@ -25,19 +25,6 @@ namespace bld {
 // helpers
 // ---------------------------------------------------------------------------
 
-// entry of block b (entries sorted by block_begin)
-__device__ __forceinline__ int find_entry(const LaunchTable &lt, int b) {
-    int lo = 0, hi = lt.n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (b >= lt.e[mid].block_begin)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
 template <typename T>
 __device__ __forceinline__ cplx<T> czero() {
     return mk(T(0), T(0));
@@ -101,6 +88,7 @@ __device__ __forceinline__ cplx<T> *run_fft(cplx<T> *a, cplx<T> *b, const FftPla
         case 32: CALL(32); break;    \
         case 48: CALL(48); break;    \
         case 64: CALL(64); break;    \
+        case 72: CALL(72); break;    \
         case 96: CALL(96); break;    \
         case 128: CALL(128); break;  \
         case 144: CALL(144); break;  \

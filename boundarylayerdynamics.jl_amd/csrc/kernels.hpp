@@ -8,7 +8,9 @@
 namespace bld {
 
 enum PassKind : int { PASS_C2R_X2 = 0, PASS_C2R_X1 = 1, PASS_R2C_X1 = 2, PASS_R2C_X2 = 3 };
-enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2 };
+enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2, PHYS_SGS_MLP = 3, PHYS_SGS_MLP_STRESS = 4 };
+// C-node points per block of the learned-closure kernel (256 threads x chunks)
+constexpr int kMlpChunks = 4;
 
 // Spectral linear terms of one model instance.
 template <typename T>
@@ -52,6 +54,13 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
                       hipStream_t st);
 template <typename T>
 void launch_physics(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, hipStream_t st);
+// learned closure (PHYS_SGS_MLP: stresses, PHYS_SGS_MLP_STRESS: their rates)
+template <typename T>
+void launch_closure(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, hipStream_t st);
+// box-filtered samples (filter width F cells) of one space's physical terms
+template <typename T>
+void launch_closure_sample(const Space<T> *space_dev, int N1, int N2, int n3, int F, double delta2, const double *xw,
+                           float *X, float *Y, hipStream_t st);
 template <typename T>
 void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st);
 template <typename T>

@@ -18,7 +18,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import BldConfig, BldError, dptr
+from ._lib import BldConfig, BldError, dptr, fptr
 
 # ---------------------------------------------------------------------------
 # Domain and boundaries (src/Domains.jl)
@@ -140,6 +140,44 @@ class StaticSmagorinskyModel:
         self.wall_damping_exponent = wall_damping_exponent
 
 
+class LearnedSGSModel:
+    """Learned SGS closure (no reference counterpart), a process next to
+    StaticSmagorinskyModel (static_smagorinsky.jl:39-42) with the same state
+    fields and rate terms: the network of `closure.MLP` gives the deviatoric
+    stress at every C-node from the normalised velocity gradient and a wall
+    feature; `fallback`, a StaticSmagorinskyModel, supplies the wall model and
+    the eddy-viscosity stress wherever the network output is not finite.
+
+    ``params``: flat float32 parameters (closure.pack layout) or a dict;
+    ``delta``: Δ of the stress scale Δ²|∇u|² (default: the geometric mean of the
+    horizontal grid spacings of the default physical grid); ``guard``: points
+    where the learned stress is not dissipative (−τ:S < 0) take the fallback's
+    stress."""
+
+    def __init__(self, params, fallback=None, delta=None, guard=True):
+        self.params = params
+        self.guard = guard
+        self.fallback = StaticSmagorinskyModel() if fallback is None else fallback
+        self.delta = delta
+        self.dealiasing = self.fallback.dealiasing
+
+
+def wall_feature(domain, n3, delta):
+    """Network input 10 per C-layer: 1/(1 + d/Δ) with d the distance to the
+    nearest wall that is not free-slip (0 without such a wall)."""
+    zc = [domain.vmap(z) for z in _zeta(n3, "C")]
+    x3min, x3max = domain.vmap(0.0), domain.vmap(1.0)
+    out = []
+    for z in zc:
+        d = math.inf
+        if not isinstance(domain.lower_boundary, FreeSlipBoundary):
+            d = min(d, z - x3min)
+        if not isinstance(domain.upper_boundary, FreeSlipBoundary):
+            d = min(d, x3max - z)
+        out.append(0.0 if math.isinf(d) else 1.0 / (1.0 + d / delta))
+    return np.array(out, dtype=np.float64)
+
+
 class SSPRK33:
     """Three-stage SSP Runge-Kutta (ODEMethods.jl:50,147-166), the engine's method."""
 
@@ -236,14 +274,30 @@ class Discretization:
         self.fields = []
         for p in flat:  # Processes.jl:326-336
             fs = ("vel1", "vel2", "vel3") if isinstance(
-                p, (MomentumAdvection, Pressure, StaticSmagorinskyModel)) else (p.field,)
+                p, (MomentumAdvection, Pressure, StaticSmagorinskyModel, LearnedSGSModel)) else (p.field,)
             for f in fs:
                 if f not in FIELD_INDEX:
                     raise NotImplementedError(f"field {f!r}: the GPU engine advances vel1, vel2, vel3")
                 if f not in self.fields:
                     self.fields.append(f)
         self._keep = []
+        self.closure = None
         self.config = self._config()
+
+    def closure_args(self):
+        """(params float32, Δ, wall feature, flags) of the learned closure, or None."""
+        p = self.closure
+        if p is None:
+            return None
+        from . import closure as _cl
+        flat = np.ascontiguousarray(p.params if isinstance(p.params, np.ndarray) else _cl.pack(p.params),
+                                    dtype=np.float32)
+        if flat.shape != (_cl.NPARAM,):
+            raise ValueError(f"learned closure: expected {_cl.NPARAM} parameters, got {flat.shape}")
+        d = self.domain
+        delta = p.delta if p.delta is not None else math.sqrt(
+            d.size(1) / (2 * self.k1max + 2) * d.size(2) / (2 * self.k2max + 2))
+        return flat, float(delta), wall_feature(d, self.n3, delta), int(bool(p.guard))
 
     def _arr(self, values):
         a = np.array(values, dtype=np.float64)
@@ -283,7 +337,14 @@ class Discretization:
             elif isinstance(p, ConstantSource):
                 c.source[FIELD_INDEX[p.field]] += float(p.strength)
             elif isinstance(p, StaticSmagorinskyModel):
+                if c.sgs:
+                    raise NotImplementedError("more than one SGS process")
                 self._config_sgs(c, p)
+            elif isinstance(p, LearnedSGSModel):
+                if c.sgs:
+                    raise NotImplementedError("more than one SGS process")
+                self._config_sgs(c, p.fallback)
+                self.closure = p
             else:
                 raise NotImplementedError(f"process {type(p).__name__} is not part of the GPU step engine")
         return c
@@ -341,6 +402,14 @@ class Model:
         if rc != 0:
             raise BldError(rc, lib.bld_last_error(None).decode())
         self._attach(disc, lib, h, 0, owner=True)
+        self._apply_closure()
+
+    def _apply_closure(self):
+        args = self.disc.closure_args()
+        if args is not None:
+            flat, delta, xw, flags = args
+            self._use()
+            self._check(self._lib.bld_sgs_closure(self._h, fptr(flat), delta, dptr(xw), flags))
 
     def _attach(self, disc, lib, h, index, owner):
         self.disc = disc
@@ -486,6 +555,25 @@ class Model:
     def set_profiling(self, on):
         self._check(self._lib.bld_set_profiling(self._h, 1 if on else 0))
 
+    def sample_closure(self, filter_width):
+        """Training samples of the current state for the learned closure: box
+        filter of `filter_width` cells in x1 and x2 on each C-layer of the
+        default physical grid (Δ = filter_width × the geometric mean spacing).
+        Returns (x, t, Δ): float32 [n, 16] features and targets (t[:, :6] the
+        normalised deviatoric stress, t[:, 15] its scale Δ²|ḡ|²)."""
+        d = self.domain
+        delta = filter_width * math.sqrt(d.size(1) / (2 * self.k1max + 2) * d.size(2) / (2 * self.k2max + 2))
+        xw = wall_feature(d, self.n3, delta)
+        n = ctypes.c_int64()
+        self._use()
+        self._check(self._lib.bld_closure_sample(self._h, filter_width, delta, dptr(xw), None, None, 0,
+                                                 ctypes.byref(n)))
+        x = np.zeros((n.value, 16), dtype=np.float32)
+        t = np.zeros((n.value, 16), dtype=np.float32)
+        self._check(self._lib.bld_closure_sample(self._h, filter_width, delta, dptr(xw), fptr(x), fptr(t), n.value,
+                                                 ctypes.byref(n)))
+        return x, t, delta
+
     def kernel_times(self):
         out = {}
         for k, name in enumerate(_lib.KERNEL_KINDS):
@@ -549,6 +637,8 @@ class Ensemble:
             raise BldError(rc, lib.bld_last_error(None).decode())
         self._lib, self._h = lib, h
         self.members = [Model._member(d, lib, h, i, self) for i, d in enumerate(self.discs)]
+        for m in self.members:
+            m._apply_closure()
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -578,6 +668,11 @@ class Ensemble:
 
     def autotune(self, reps=5):
         self._check(self._lib.bld_autotune(self._h, reps))
+
+    def set_profiling(self, on):
+        self._check(self._lib.bld_set_profiling(self._h, 1 if on else 0))
+
+    kernel_times = Model.kernel_times
 
     def grid_points(self):
         return sum(d.n3 * (2 * d.k1max + 2) * (2 * d.k2max + 2) for d in self.discs)

@@ -11,14 +11,36 @@ import math
 import numpy as np
 
 
-def les_neutral(M, n=(128, 128, 64), Cs=0.1):
+def les_neutral(M, n=(128, 128, 64), Cs=0.1, fft_friendly=False):
     """Neutral atmospheric half channel LES: 2π×2π×1, rough wall z0=1e-4, κ=0.4,
     free-slip top, static Smagorinsky with Mason wall damping, ν=1e-5,
-    constant mean pressure gradient forcing 1 (les_test.jl:55-58 style)."""
+    constant mean pressure gradient forcing 1 (les_test.jl:55-58 style).
+
+    ``fft_friendly`` sets the advection grid through the reference's tuple
+    dealiasing (physical_space.jl:45) to the smallest 2^a·3^b ≥ 3·kmax+1
+    (alias-free for the quadratic products) instead of the `:quadratic`
+    heuristic, which gives sizes such as 142 = 2·71 for kmax = 47."""
     dom = M.Domain((2 * math.pi, 2 * math.pi, 1.0), M.RoughWall(1e-4, 0.4), M.FreeSlipBoundary())
     procs = M.incompressible_flow(1e-5, constant_forcing=1,
                                   sgs_model=M.StaticSmagorinskyModel(Cs=Cs, wall_damping=True))
+    if fft_friendly:
+        sizes = tuple(smooth_size(3 * ((k - 1) // 2) + 1) for k in n[:2])
+        procs[0] = M.MomentumAdvection(dealiasing=sizes)
     return n, dom, procs
+
+
+def smooth_size(nmin):
+    """Smallest 2^a·3^b ≥ nmin."""
+    best = None
+    a = 1
+    while a < 4 * nmin:
+        b = a
+        while b < 4 * nmin:
+            if b >= nmin and (best is None or b < best):
+                best = b
+            b *= 3
+        a *= 2
+    return best
 
 
 def les_initial(domain_z, n1, n2, seed=7, z0=1e-4, kappa=0.4, sigma=0.1):

@@ -120,6 +120,27 @@ int bld_autotune(bld_engine *e, int reps);
  * 2 r2c_x1, 3 r2c_x2) for the entry-th model space. */
 int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us);
 
+/* Learned SGS closure (no reference counterpart; it replaces the stresses of
+ * the selected model's StaticSmagorinskyModel, static_smagorinsky.jl:129-249,
+ * which stays configured as its wall model and per-point fallback).  params:
+ * bld_mlp_nparams() floats in the bld_mlp layout; delta: Δ of the stress scale
+ * Δ²|∇u|²; wall_feature: network input 10 per C-layer [n3].  params = NULL
+ * restores the Smagorinsky stresses.  Features: ∇u/|∇u| at each C-node
+ * (row-major ∂u_i/∂x_j), then the wall feature; outputs τ^dev/(Δ²|∇u|²) for
+ * (11, 22, 33, 12, 13, 23).  Points where the output is not finite or ∇u
+ * vanishes take the Smagorinsky stress; with BLD_CLOSURE_GUARD in `flags` so
+ * do points where the learned stress is not dissipative (−τ:S < 0). */
+#define BLD_CLOSURE_GUARD 1
+int bld_sgs_closure(bld_engine *e, const float *params, double delta, const double *wall_feature, int flags);
+/* Training samples of the selected model's current state: box filter of
+ * `filter` grid cells in x1 and x2 on each C-layer of the default physical grid
+ * (which `filter` must divide), one sample per filter box and layer.  x, t:
+ * float32 [count][16] (features as above; t[0..5] = τ^dev/(Δ²|ḡ|²) with
+ * τ_ij = avg(u_i u_j) − avg(u_i)·avg(u_j), t[15] = Δ²|ḡ|²).  With x or t NULL
+ * only *count is set. */
+int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wall_feature, float *x, float *t,
+                       int64_t capacity, int64_t *count);
+
 /* Message of the last failed call on `e` (or of the last failed bld_create
  * on this thread when e is NULL). */
 const char *bld_last_error(const bld_engine *e);
