@@ -77,13 +77,18 @@ def roofline(B, m, kt, precision, res, adv_n, sgs_n):
     sizes = dict(K1=m.k1max + 1, K2=2 * m.k2max + 1, n3=m.n3,
                  spaces=[("adv", *adv_n), ("sgs", *sgs_n)])
     ab = B.workloads.algorithmic_bytes(sizes, None, precision)
+    # ab[k]: algorithmic bytes of kernel class k per right-hand side (all spaces);
+    # a class may take several dispatches per right-hand side (one per space or
+    # LDS class), so bytes and durations are averaged per dispatch
+    n_rhs = kt["linear"][1]
     per = {}
     for k, (ms, n) in kt.items():
         if n == 0 or k not in ab:
             continue
-        avg_ms = ms / n
-        per[k] = dict(launches=n, avg_us=1e3 * avg_ms, total_ms=ms, bytes_per_launch=ab[k],
-                      gbs=ab[k] / (avg_ms * 1e-3) / 1e9)
+        calls = n_rhs if k != "stage_project" else n
+        total_bytes = ab[k] * calls
+        per[k] = dict(launches=n, avg_us=1e3 * ms / n, total_ms=ms, bytes_per_launch=total_bytes / n,
+                      gbs=total_bytes / (ms * 1e-3) / 1e9)
     dom = max(per, key=lambda k: per[k]["total_ms"])
     d = per[dom]
     return dict(kernel=dom, bound="hbm", achieved=round(d["gbs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
@@ -201,6 +206,156 @@ def ensemble_bench(B, dt, steps, warmup, autotune):
                 autotuned=autotune, finite=finite, kernel_ms_per_step=kt)
 
 
+MLP_FLOP_PER_POINT = 2 * (10 * 64 + 64 * 64 + 64 * 6)  # forward, useful (unpadded) multiply-adds
+BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+def closure_bench(B, res, dt, steps, warmup):
+    """The same LES with the learned SGS closure (bf16 MFMA network at every
+    C-node, Smagorinsky fallback and dissipation guard) in fp32, plus the
+    network's standalone forward / training-gradient throughput."""
+    W, CL = B.workloads, B.closure
+    n, dom, procs = W.les_neutral(B, res)
+    procs[-1] = B.LearnedSGSModel(CL.pack(CL.init_params(3)), fallback=procs[-1], guard=True)
+    m = B.Model(n, dom, procs, precision="fp32")
+    n1, n2 = m.default_size()
+    u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2)
+    m.set_physical("vel1", u)
+    m.set_physical("vel2", v)
+    m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
+    m.init_problem()
+    m.step(dt, warmup)
+    ms = m.time_steps(dt, steps)
+    m.set_profiling(True)
+    m.time_steps(dt, 5, graph=False)
+    kt = m.kernel_times()
+    m.set_profiling(False)
+    finite = all(np.all(np.isfinite(a)) for a in m.state.values())
+    ms_mlp, launches = kt["sgs_mlp"]
+    pts = n1 * n2 * m.n3
+    tflops = MLP_FLOP_PER_POINT * pts / (ms_mlp / launches * 1e-3) / 1e12
+    net = CL.MLP(seed=1)
+    npts = 1 << 20
+    f_ms, g_ms = net.time(npts, reps=10)
+    return dict(steps_per_s=round(1e3 * steps / ms, 2), ms_per_step=round(ms / steps, 4), finite=finite,
+                sgs_mlp_avg_us=round(1e3 * ms_mlp / launches, 2), points_per_launch=pts,
+                sgs_mlp_tflops=round(tflops, 2), sgs_mlp_frac_bf16_peak=round(tflops / BF16_DENSE_PEAK_TFLOPS, 4),
+                kernels={k: round(1e3 * v[0] / v[1], 2) for k, v in kt.items() if v[1]},
+                mlp_forward_1M_ms=round(f_ms, 4),
+                mlp_forward_tflops=round(MLP_FLOP_PER_POINT * npts / (f_ms * 1e-3) / 1e12, 2),
+                mlp_gradient_1M_ms=round(g_ms, 4),
+                mlp_gradient_tflops=round(3 * MLP_FLOP_PER_POINT * npts / (g_ms * 1e-3) / 1e12, 2))
+
+
+def copy_probe(mib=256, reps=10):
+    """A plain device-to-device copy of known size (torch's copy kernel), the
+    scale for the FETCH_SIZE / WRITE_SIZE counters: each dispatch reads and
+    writes exactly `mib` MiB."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "reps": reps}), flush=True)
+
+
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+    rows = []
+    for path in sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True)):
+        with open(path, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _short(name):
+    return name.replace("void ", "").replace("bld::", "").split("(")[0]
+
+
+def _last_half(rows, key):
+    """The later half of a kernel's dispatches (the timed steps come after the
+    autotuner's trial launches and make up most of the run)."""
+    rows = sorted(rows, key=key)
+    return rows[len(rows) // 2:]
+
+
+def _counter_per_kernel(directory, counter):
+    """Mean of `counter` (KiB per dispatch) per kernel over the later half of
+    its dispatches."""
+    rows = [r for r in _csv_rows(directory, "counter_collection.csv") if r.get("Counter_Name") == counter]
+    by = {}
+    for r in rows:
+        by.setdefault(_short(r["Kernel_Name"]), []).append(r)
+    out = {}
+    for k, rs in by.items():
+        vals = [float(r["Counter_Value"]) for r in _last_half(rs, lambda r: int(r.get("Dispatch_Id", 0) or 0))]
+        out[k] = (sum(vals) / len(vals), len(vals))
+    return out
+
+
+def rocprof_summary(kt_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes):
+    """Markdown summary of a rocprofv3 --kernel-trace --stats run and of the
+    FETCH_SIZE / WRITE_SIZE runs, the counters scaled by what they read for the
+    copy probe (copy_bytes read and written per dispatch)."""
+    lines = ["| kernel | calls | average us | total us | % |", "|---|---|---|---|---|"]
+    for r in sorted(_csv_rows(kt_dir, "kernel_stats.csv"), key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"| {_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['TotalDurationNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    # per-dispatch trace: the timed configuration only (grid size of each
+    # kernel's last dispatch), without the autotuner's trial launches
+    trace = _csv_rows(kt_dir, "kernel_trace.csv")
+    by = {}
+    for r in trace:
+        by.setdefault(_short(r["Kernel_Name"]), []).append(r)
+    lines += ["", "Timed configuration (the later half of each kernel's dispatches, after the autotuner's "
+                  "trial launches):", "",
+              "| kernel | dispatches | grid sizes | average us |", "|---|---|---|---|"]
+    grid_of = lambda r: r.get("Grid_Size") or r.get("Grid_Size_X")
+    for k, rs in sorted(by.items(), key=lambda kv: -sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+                                                         for r in kv[1])):
+        late = _last_half(rs, lambda r: int(r["Start_Timestamp"]))
+        d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in late]
+        grids = sorted({int(grid_of(r)) for r in late})
+        lines.append(f"| {k} | {len(d)} | {' '.join(map(str, grids))} | {sum(d) / len(d) / 1e3:.2f} |")
+    cf = _counter_per_kernel(copy_fetch_dir, "FETCH_SIZE")
+    cw = _counter_per_kernel(copy_write_dir, "WRITE_SIZE")
+    ck = max(cf, key=lambda k: cf[k][0])
+    kib = copy_bytes / 1024
+    sf, sw = kib / cf[ck][0], kib / cw[max(cw, key=lambda k: cw[k][0])][0]
+    lines += ["", f"Copy probe ({copy_bytes / 2 ** 20:.0f} MiB read + written per dispatch, kernel `{ck}`): "
+                  f"FETCH_SIZE {cf[ck][0]:.0f} KiB, scale {sf:.4f}; WRITE_SIZE "
+                  f"{cw[max(cw, key=lambda k: cw[k][0])][0]:.0f} KiB, scale {sw:.4f}.", "",
+              "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic MB (scaled) |", "|---|---|---|---|---|"]
+    f = _counter_per_kernel(fetch_dir, "FETCH_SIZE")
+    w = _counter_per_kernel(write_dir, "WRITE_SIZE")
+    for k in sorted(f, key=lambda k: -f[k][0]):
+        if k in w:
+            mb = (f[k][0] * sf + w[k][0] * sw) * 1024 / 1e6
+            lines.append(f"| {k} | {f[k][1]} | {f[k][0]:.0f} | {w[k][0]:.0f} | {mb:.2f} |")
+    print("\n".join(lines))
+
+
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_les128x128x64_fp32.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (the 'traffic MB (scaled)' column), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith("k_" + kernel + "<"):
+            try:
+                return int(float(cells[4]) * 1e6)
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,12 +369,22 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--autotune", type=int, default=1)
+    ap.add_argument("--copy-probe", action="store_true", help="only run the counter-scale copy kernel")
+    ap.add_argument("--rocprof-summary", nargs=5, metavar="DIR",
+                    help="summarise rocprofv3 output: kernel-trace, FETCH_SIZE, WRITE_SIZE, copy FETCH, copy WRITE")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        rocprof_summary(*args.rocprof_summary, copy_bytes=256 * 2 ** 20)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path "
               f"in this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     res = tuple(int(x) for x in args.res.split(","))
     B = load_package()
     log("fp32 mixed form ...")
@@ -229,6 +394,8 @@ def main():
     adv_n = B.pdsize((m32.k1max, m32.k2max), "quadratic")
     sgs_n = B.pdsize((m32.k1max, m32.k2max), None)
     roof, per32 = roofline(B, m32, kt32, "fp32", res, adv_n, sgs_n)
+    if res == (128, 128, 64):
+        roof["traffic"] = measured_traffic(roof["kernel"])
     log("fp32:", r32, "dominant", roof["kernel"])
     fp64 = None
     if not args.no_fp64:
@@ -247,6 +414,11 @@ def main():
         log("ensemble ...")
         ens = ensemble_bench(B, args.dt, max(args.steps // 10, 10), 5, bool(args.autotune))
         log("ensemble:", ens)
+    clo = None
+    if not args.no_closure:
+        log("learned closure ...")
+        clo = closure_bench(B, res, args.dt, max(args.steps // 3, 10), 5)
+        log("closure:", clo)
     cpu = None if args.no_cpu else cpu_baseline(res, args.dt, args.cpu_seconds)
     sps = 1e3 / r32["ms_per_step"]
     points = res[0] * res[1] * res[2]
@@ -280,6 +452,7 @@ def main():
         "fft_autotune": tuning32,
         "pressure_solve_65536x256": pres,
         "ensemble_32_uneven_les": ens,
+        "learned_closure_les": clo,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_sgs_mlp(const Space<T> *__restrict__ sp
         float g[9], gn2 = 0.f;
         int l = 0;
         if (live) {
-            l = int(gid / P);
+            l = int(uint32_t(gid) / uint32_t(P));
             velocity_gradient<T, float>(S, P, l, gid - size_t(l) * P, g);
 #pragma unroll
             for (int k = 0; k < 9; ++k) gn2 += g[k] * g[k];
@@ -165,7 +165,7 @@ __global__ void __launch_bounds__(256) k_sgs_mlp_stress(const Space<T> *__restri
     const size_t P = size_t(S.N1) * S.N2;
     const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
-    const int l = int(gid / P), n3 = S.n3;
+    const int l = int(uint32_t(gid) / uint32_t(P)), n3 = S.n3;
     const size_t x = gid - size_t(l) * P;
     S.src_phys[3][gid] = -S.tau[0][gid];
     S.src_phys[4][gid] = -S.tau[1][gid];

@@ -12,6 +12,7 @@
 #include "bld.h"
 #include "kernels.hpp"
 #include "mlp.hpp"
+#include "transforms_reg.hpp"
 
 namespace bld {
 
@@ -229,6 +230,9 @@ class Engine : public EngineBase {
             if (cfgs[i].precision != cfgs[0].precision)
                 throw BldError{BLD_EARG, "all models of an ensemble share one precision"};
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         d_flag.alloc(sizeof(int));
         HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
         for (int i = 0; i < n; ++i) {
@@ -245,10 +249,17 @@ class Engine : public EngineBase {
     ~Engine() override {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (auto &ev : events) (void)hipEventDestroy(ev);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 
     hipStream_t st = nullptr;
+    // side stream of the float64 (0,0)-mode stage pass, forked from and joined
+    // back into `st` (also inside the captured graph)
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::vector<std::unique_ptr<Instance<T>>> inst;
     int sel_inst = 0;
     TwiddleSet<T> tw;
@@ -273,6 +284,7 @@ class Engine : public EngineBase {
     Pass pp[5];  // physics: advection products, SGS viscosity, SGS stress, learned stress, its rates
     Pass lin_pass, stage_pass, stage00_pass, xfer[4];
     int stage_CL = 4;
+    int stage_CL_tuned = 0;  // chunk length chosen by the autotuner (0: by n3)
     // tuning results: per transform pass and entry
     std::vector<double> tuned_us[5];
     // graph
@@ -467,6 +479,8 @@ class Engine : public EngineBase {
 
     int add_space(Instance<T> &I, int kind, int N1, int N2) {
         const int n3 = I.n3;
+        if (double(N1) * N2 * n3 * 10 >= 2147483648.0 || double(I.nk) * (3.0 * n3) >= 2147483648.0)
+            throw BldError{BLD_EUNSUPPORTED, "grid too large for 32-bit point indices"};
         Space<T> S;
         std::memset(&S, 0, sizeof(S));
         S.kind = kind;
@@ -605,9 +619,16 @@ class Engine : public EngineBase {
     }
 
     // default batch: keep one block's LDS at <= 64 KiB so several blocks share a CU
-    static int default_batch(int n, int bufs, int avail) {
+    // LDS bytes per batch item (column or row pair) of a transform pass: the
+    // register FFT needs one exchange buffer, the LDS FFT ping-pong buffers
+    static size_t item_lds(int kind, int n, int bufs) {
+        const int e = reg_fft_a(n) > 0 ? reg_item_lds(kind, n, bufs / 2) : bufs * n;
+        return size_t(e) * sizeof(cplx<T>);
+    }
+
+    static int default_batch(int kind, int n, int bufs, int avail) {
         const size_t limit = 64 * 1024;
-        int b = int(limit / (size_t(bufs) * n * sizeof(cplx<T>)));
+        int b = int(limit / item_lds(kind, n, bufs));
         b = std::max(1, std::min(b, 16));
         b = std::min(b, std::max(1, avail));
         int p2 = 1;  // the kernels split indices with shifts: keep B a power of two
@@ -621,14 +642,14 @@ class Engine : public EngineBase {
         pass_geometry(kind, S, n, bufs, avail, njobs);
         LaunchEntry E{};
         E.space = sidx;
-        E.batch = batch > 0 ? batch : default_batch(n, bufs, avail);
+        E.batch = batch > 0 ? batch : default_batch(kind, n, bufs, avail);
         E.blocks_per_job = (avail + E.batch - 1) / E.batch;
         E.njobs = njobs;
         E.block_begin = ps.nblocks;
         ps.ent.push_back(E);
         ps.kind.push_back(kind);
         ps.nblocks += E.blocks_per_job * njobs;
-        const size_t lds = size_t(bufs) * E.batch * n * sizeof(cplx<T>);
+        const size_t lds = item_lds(kind, n, bufs) * E.batch;
         if (lds > 160 * 1024) throw BldError{BLD_EUNSUPPORTED, "transform length too large for LDS"};
         ps.smem = std::max(ps.smem, lds);
     }
@@ -648,11 +669,17 @@ class Engine : public EngineBase {
 
     static int lds_class(size_t lds) { return lds <= 16 * 1024 ? 0 : (lds <= 32 * 1024 ? 1 : 2); }
 
+    // ensembles span LDS footprints several times apart: one launch per LDS
+    // class; a single model's spaces share one launch
     void bucket_add(std::vector<Pass> &v, int kind, int sidx, int batch) {
         if (v.empty()) v.resize(3);
-        Pass probe;
-        add_entry(probe, kind, sidx, batch);
-        add_entry(v[lds_class(probe.smem)], kind, sidx, batch);
+        int cls = 0;
+        if (inst.size() > 1) {
+            Pass probe;
+            add_entry(probe, kind, sidx, batch);
+            cls = lds_class(probe.smem);
+        }
+        add_entry(v[cls], kind, sidx, batch);
     }
 
     void bucket_finish(std::vector<Pass> &v) {
@@ -683,7 +710,7 @@ class Engine : public EngineBase {
             for (int k = 0; k < 4; ++k) {
                 int n, bufs, avail, njobs;
                 pass_geometry(k, spaces[s], n, bufs, avail, njobs);
-                tp_entry_space[k].push_back({s, b(k) > 0 ? b(k) : default_batch(n, bufs, avail)});
+                tp_entry_space[k].push_back({s, b(k) > 0 ? b(k) : default_batch(k, n, bufs, avail)});
             }
         }
         for (int k = 0; k < 4; ++k) bucket_finish(tp[k]);
@@ -736,7 +763,7 @@ class Engine : public EngineBase {
         // chunked stage kernel: CL shared by the launch, NC = pow2 ≥ ⌈n3/CL⌉, MB = 256/NC
         int n3max = 0;
         for (auto &I : inst) n3max = std::max(n3max, I->n3);
-        stage_CL = n3max <= 64 ? 4 : (n3max <= 128 ? 8 : 16);
+        stage_CL = stage_CL_tuned ? stage_CL_tuned : (n3max <= 64 ? 4 : (n3max <= 128 ? 8 : 16));
         reset(stage_pass);
         reset(stage00_pass);
         std::vector<ProjDesc<T>> projs;
@@ -744,10 +771,8 @@ class Engine : public EngineBase {
         for (auto &I : inst) {
             int nc = (I->n3 + stage_CL - 1) / stage_CL, NC = 1;
             while (NC < nc) NC *= 2;
-            if (NC > 16) {
-                I->sequential_stage = true;
-                continue;
-            }
+            I->sequential_stage = NC > 64;
+            if (I->sequential_stage) continue;
             I->proj.MB = 256 / NC;
             LaunchEntry E{};
             E.space = int(projs.size());
@@ -811,12 +836,9 @@ class Engine : public EngineBase {
         timed(kk, [&] { launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt(), p.nblocks, p.smem, sel, st); });
     }
 
+    // one timed dispatch per LDS class
     void run_transform(int kind, const std::vector<Pass> &v, int sel, int kk) {
-        if (v.empty()) return;
-        timed(kk, [&] {
-            for (const Pass &p : v)
-                if (p.nblocks) launch_transform<T>(kind, d_spaces.as<Space<T>>(), p.lt(), p.nblocks, p.smem, sel, st);
-        });
+        for (const Pass &p : v) run_transform(kind, p, sel, kk);
     }
 
     // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du
@@ -836,13 +858,20 @@ class Engine : public EngineBase {
         run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
     }
 
+    // The float64 (0,0) pass of the mixed form touches only the (0,0) columns,
+    // which the main pass skips: it runs concurrently on the side stream.
     void stage(int s, double dt) {
         timed(KK_STAGE, [&] {
+            if (stage00_pass.nblocks) {
+                HIPCHECK(hipEventRecord(ev_fork, st));
+                HIPCHECK(hipStreamWaitEvent(st2, ev_fork, 0));
+                launch_stage_chunked<double>(stage_CL, d_proj00.as<ProjDesc<double>>(), stage00_pass.lt(),
+                                             stage00_pass.nblocks, s, dt, st2);
+                HIPCHECK(hipEventRecord(ev_join, st2));
+            }
             launch_stage_chunked<T>(stage_CL, d_proj.as<ProjDesc<T>>(), stage_pass.lt(), stage_pass.nblocks, s, dt,
                                     st);
-            if (stage00_pass.nblocks)
-                launch_stage_chunked<double>(stage_CL, d_proj00.as<ProjDesc<double>>(), stage00_pass.lt(),
-                                             stage00_pass.nblocks, s, dt, st);
+            if (stage00_pass.nblocks) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
             for (auto &I : inst)
                 if (I->sequential_stage) launch_stage_sequential<T>(I->proj, s, dt, st);
         });
@@ -1050,7 +1079,7 @@ class Engine : public EngineBase {
                 pass_geometry(k, spaces[s], n, bufs, avail, njobs);
                 double bt = 1e30;
                 for (int b = 1; b <= 32 && b <= std::max(1, avail); b *= 2) {
-                    if (size_t(bufs) * b * n * sizeof(cplx<T>) > 64 * 1024) break;
+                    if (item_lds(k, n, bufs) * b > 64 * 1024) break;
                     const double us = time_entry(k, s, b, 0, reps);
                     if (us < bt) {
                         bt = us;
@@ -1061,6 +1090,36 @@ class Engine : public EngineBase {
             }
         }
         build_transform_passes(&best);
+        // chunk length of the stage kernel: stage 1 only writes utmp, so it can be
+        // replayed on the live state
+        int n3max = 0;
+        for (auto &I : inst) n3max = std::max(n3max, I->n3);
+        double best_us = 1e30;
+        int best_cl = 0;
+        for (int cl : {2, 4, 8, 16}) {
+            if (n3max > 64 * cl) continue;
+            stage_CL_tuned = cl;
+            build_point_plans();
+            stage(1, 1e-6);
+            hipEvent_t a, b;
+            HIPCHECK(hipEventCreate(&a));
+            HIPCHECK(hipEventCreate(&b));
+            HIPCHECK(hipEventRecord(a, st));
+            for (int r = 0; r < reps; ++r) stage(1, 1e-6);
+            HIPCHECK(hipEventRecord(b, st));
+            HIPCHECK(hipEventSynchronize(b));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, a, b));
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            if (1e3 * ms / reps < best_us) {
+                best_us = 1e3 * ms / reps;
+                best_cl = cl;
+            }
+        }
+        stage_CL_tuned = best_cl;
+        tuned_us[4].assign(1, best_us);
+        build_point_plans();
         if (graph_exec) {
             HIPCHECK(hipGraphExecDestroy(graph_exec));
             graph_exec = nullptr;
@@ -1070,6 +1129,11 @@ class Engine : public EngineBase {
     }
 
     void tuning(int pass, int entry, int *batch, double *us) override {
+        if (pass == 4) {  // stage kernel: chunk length and one stage's time
+            *batch = stage_CL;
+            *us = tuned_us[4].empty() ? 0.0 : tuned_us[4][0];
+            return;
+        }
         if (pass < 0 || pass > 3) throw BldError{BLD_EARG, "unknown pass"};
         const auto &v = tp_entry_space[pass];
         if (entry < 0 || entry >= int(v.size())) throw BldError{BLD_EARG, "entry index out of range"};

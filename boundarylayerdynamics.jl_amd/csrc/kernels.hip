@@ -18,58 +18,14 @@
 #include "kernels.hpp"
 
 #include "fft.hpp"
+#include "terms.hpp"
+#include "transforms_reg.hpp"
 
 namespace bld {
 
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-
-template <typename T>
-__device__ __forceinline__ cplx<T> czero() {
-    return mk(T(0), T(0));
-}
-
-// Spectral value of a c2r term at (k1, i2) of `layer` (derivatives.jl:404-418,566-568).
-template <typename T>
-__device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int recipe, int sel, int layer, int k1,
-                                              int i2) {
-    const size_t P = size_t(S.K1) * S.K2;
-    const size_t idx = size_t(layer) * P + k1 + size_t(S.K1) * i2;
-    const cplx<T> *u = S.state[sel][0], *v = S.state[sel][1], *w = S.state[sel][2];
-    const T d1 = T(S.D1[k1]), d2 = T(S.D2[i2]);
-    switch (recipe) {
-        case R_U: return u[idx];
-        case R_V: return v[idx];
-        case R_W: return w[idx];
-        case R_D1U: return imul(d1, u[idx]);
-        case R_D2U: return imul(d2, u[idx]);
-        case R_D1V: return imul(d1, v[idx]);
-        case R_D2V: return imul(d2, v[idx]);
-        case R_D1W: return imul(d1, w[idx]);
-        case R_D2W: return imul(d2, w[idx]);
-        case R_VORT1: {  // D2·w - (-α·v¯ + α·v⁺)
-            const T a = T(S.alpha_i[layer + 1]);
-            cplx<T> dz = (-a) * v[idx] + a * v[idx + P];
-            return imul(d2, w[idx]) - dz;
-        }
-        case R_VORT2: {  // (-α·u¯ + α·u⁺) - D1·w
-            const T a = T(S.alpha_i[layer + 1]);
-            cplx<T> dz = (-a) * u[idx] + a * u[idx + P];
-            return dz - imul(d1, w[idx]);
-        }
-        case R_VORT3: return imul(d1, v[idx]) - imul(d2, u[idx]);
-        default: return S.in_field[term][idx];
-    }
-}
-
-// Division by a block-uniform runtime divisor d (n·d < 2^32): q = ⌊n·m/2^32⌋
-// with m = ⌈2^32/d⌉.
-struct Div {
-    uint32_t d, m;
-    __device__ explicit Div(int dd) : d(uint32_t(dd)), m(uint32_t(0xFFFFFFFFu / uint32_t(dd)) + 1u) {}
-    __device__ __forceinline__ int q(int n) const { return d == 1 ? n : int(__umulhi(uint32_t(n), m)); }
-};
 
 template <typename T, bool INV, int FN>
 __device__ __forceinline__ cplx<T> *run_fft(cplx<T> *a, cplx<T> *b, const FftPlan &p, int batch,
@@ -112,12 +68,12 @@ __device__ __forceinline__ void c2r_x2_body(const Space<T> &S, const LaunchEntry
     const int recipe = S.in_recipe[term];
     const int B = E.batch;
     const int N2 = FN > 0 ? FN : S.N2;
-    const Div divN(N2);
     cplx<T> *bufA = smem, *bufB = smem + B * N2;
     const int k1_0 = chunk * B;
     const int ncols = min(B, S.nk1 - k1_0);
+    const int lb = __ffs(B) - 1;  // B is a power of two
     for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
-        const int c = divN.q(idx), pos = idx - c * N2;
+        const int pos = idx >> lb, c = idx & (B - 1);  // c fastest: coalesced over k1
         cplx<T> val = czero<T>();
         if (c < ncols) {
             int i2 = -1;
@@ -127,12 +83,11 @@ __device__ __forceinline__ void c2r_x2_body(const Space<T> &S, const LaunchEntry
                 i2 = S.K2 - (N2 - pos);
             if (i2 >= 0) val = term_value(S, term, recipe, sel, layer, k1_0 + c, i2);
         }
-        bufA[idx] = val;
+        bufA[c * N2 + pos] = val;
     }
     __syncthreads();
     cplx<T> *res = run_fft<T, true, FN>(bufA, bufB, S.p2, B, S.tw + S.p2.tw);
     cplx<T> *out = S.inter + size_t(job) * N2 * S.nk1;
-    const int lb = __ffs(B) - 1;  // B is a power of two
     for (int idx = threadIdx.x; idx < B * N2; idx += blockDim.x) {
         const int x2 = idx >> lb, c = idx & (B - 1);
         if (c >= ncols) continue;
@@ -148,7 +103,11 @@ __global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spa
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-#define BLD_CALL(n) c2r_x2_body<T, n>(S, E, sel, smem)
+#define BLD_CALL(n)                                   \
+    if constexpr (reg_fft_a(n) > 0)                   \
+        c2r_x2_reg<T, n>(S, E, sel, smem);                 \
+    else                                              \
+        c2r_x2_body<T, n>(S, E, sel, smem)
     BLD_FFT_SWITCH(S.N2, BLD_CALL)
 #undef BLD_CALL
 }
@@ -215,7 +174,11 @@ __global__ void __launch_bounds__(256) k_c2r_x1(const Space<T> *__restrict__ spa
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-#define BLD_CALL(n) c2r_x1_body<T, n>(S, E, smem)
+#define BLD_CALL(n)                                   \
+    if constexpr (reg_fft_a(n) > 0)                   \
+        c2r_x1_reg<T, n>(S, E, smem);                 \
+    else                                              \
+        c2r_x1_body<T, n>(S, E, smem)
     BLD_FFT_SWITCH(S.N1, BLD_CALL)
 #undef BLD_CALL
 }
@@ -268,7 +231,11 @@ __global__ void __launch_bounds__(256) k_r2c_x1(const Space<T> *__restrict__ spa
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-#define BLD_CALL(n) r2c_x1_body<T, n>(S, E, smem)
+#define BLD_CALL(n)                                   \
+    if constexpr (reg_fft_a(n) > 0)                   \
+        r2c_x1_reg<T, n>(S, E, smem);                 \
+    else                                              \
+        r2c_x1_body<T, n>(S, E, smem)
     BLD_FFT_SWITCH(S.N1, BLD_CALL)
 #undef BLD_CALL
 }
@@ -335,7 +302,11 @@ __global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spa
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-#define BLD_CALL(n) r2c_x2_body<T, n>(S, E, smem)
+#define BLD_CALL(n)                                   \
+    if constexpr (reg_fft_a(n) > 0)                   \
+        r2c_x2_reg<T, n>(S, E, smem);                 \
+    else                                              \
+        r2c_x2_body<T, n>(S, E, smem)
     BLD_FFT_SWITCH(S.N2, BLD_CALL)
 #undef BLD_CALL
 }
@@ -350,7 +321,7 @@ __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict
     const size_t P = size_t(S.N1) * S.N2;
     const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
-    const int l = int(gid / P);
+    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
     const size_t x = gid - size_t(l) * P;
     const T *u = S.in_phys[0], *v = S.in_phys[1], *w = S.in_phys[2];
     const T *o1 = S.in_phys[3], *o2 = S.in_phys[4], *o3 = S.in_phys[5];
@@ -412,7 +383,7 @@ __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restric
     const size_t P = size_t(S.N1) * S.N2;
     const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
-    const int l = int(gid / P);
+    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
     const size_t x = gid - size_t(l) * P;
     SgsPoint<T> q(S, P, x);
     const int n3 = S.n3;
@@ -463,7 +434,7 @@ __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__
     const size_t P = size_t(S.N1) * S.N2;
     const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     if (gid >= P * S.n3) return;
-    const int l = int(gid / P);
+    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
     const size_t x = gid - size_t(l) * P;
     SgsPoint<T> q(S, P, x);
     const int n3 = S.n3;
@@ -569,7 +540,7 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
     } else {
         return;
     }
-    const int l = int(r / nk), m = int(r - size_t(l) * nk);
+    const int l = int(uint32_t(r) / uint32_t(nk)), m = int(r - size_t(l) * nk);
     const int nl = f < 2 ? n3 : n3 - 1;
     const cplx<T> *q = L.state[sel][f];
     cplx<T> *du = L.du[f];
@@ -1014,7 +985,9 @@ void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTabl
                           double dt, hipStream_t st) {
     if (nblocks <= 0) return;
     const size_t smem = 256 * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<T>));
-    if (CL == 4)
+    if (CL == 2)
+        k_stage_chunked<T, 2><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
+    else if (CL == 4)
         k_stage_chunked<T, 4><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
     else if (CL == 8)
         k_stage_chunked<T, 8><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);

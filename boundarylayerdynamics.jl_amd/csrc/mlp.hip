@@ -1,5 +1,6 @@
 // Learned SGS closure: bfloat16 MFMA forward, forward+backward (per-wave
 // partial gradients) and a reduction + Adam kernel; C ABI (bld_mlp_*).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -54,8 +55,9 @@ __global__ void __launch_bounds__(256) k_forward(const unsigned short *__restric
     }
 }
 
-// forward + backward of the mean squared error over the 6 outputs; each wave
-// writes its partial gradient (NPARAM floats) and partial loss
+// forward + backward of the mean squared error over the 6 outputs; the waves
+// of a block sum their gradients in LDS and the block writes one partial
+// gradient (NPARAM floats) and one partial loss
 __global__ void __launch_bounds__(256) k_gradient(const unsigned short *__restrict__ wb, const float *__restrict__ p,
                                                   const float *__restrict__ X, const float *__restrict__ Tg,
                                                   int64_t n, float scale, float *__restrict__ part,
@@ -63,6 +65,8 @@ __global__ void __launch_bounds__(256) k_gradient(const unsigned short *__restri
     __shared__ unsigned short sW[NWB];
     __shared__ float sB[2 * NH + PAD];
     __shared__ unsigned short sT[WAVES][16 * NH];
+    __shared__ float sAcc[NPARAM + 1];  // block sums of the gradient and (last entry) the loss
+    for (int i = threadIdx.x; i <= NPARAM; i += blockDim.x) sAcc[i] = 0.f;
     load_params(wb, p, sW, sB);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
@@ -158,18 +162,17 @@ __global__ void __launch_bounds__(256) k_gradient(const unsigned short *__restri
             gW1[t1] = mfma(xd, dz1b, gW1[t1]);
         }
     }
-    // write this wave's partial gradient (D layouts → parameter offsets)
-    const int64_t w = int64_t(blockIdx.x) * WAVES + wave;
-    float *out = part + w * NPARAM;
+    // add this wave's gradient into the block sums (D layouts → parameter offsets)
+    float *out = sAcc;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) out[OW1 + (4 * g + j) * NH + 16 * nt + r] = gW1[nt][j];
+        for (int nt = 0; nt < 4; ++nt) atomicAdd(&out[OW1 + (4 * g + j) * NH + 16 * nt + r], gW1[nt][j]);
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
-            for (int nt = 0; nt < 4; ++nt) out[OW2 + (16 * mt + 4 * g + j) * NH + 16 * nt + r] = gW2[mt][nt][j];
-            out[OW3 + (16 * mt + 4 * g + j) * PAD + r] = gW3[mt][j];
+            for (int nt = 0; nt < 4; ++nt) atomicAdd(&out[OW2 + (16 * mt + 4 * g + j) * NH + 16 * nt + r], gW2[mt][nt][j]);
+            atomicAdd(&out[OW3 + (16 * mt + 4 * g + j) * PAD + r], gW3[mt][j]);
         }
     }
     // bias gradients: sum the four lane groups
@@ -186,15 +189,19 @@ __global__ void __launch_bounds__(256) k_gradient(const unsigned short *__restri
     if (g == 0) {
 #pragma unroll
         for (int t2 = 0; t2 < 4; ++t2) {
-            out[OB1 + 16 * t2 + r] = gb1[t2];
-            out[OB2 + 16 * t2 + r] = gb2[t2];
+            atomicAdd(&out[OB1 + 16 * t2 + r], gb1[t2]);
+            atomicAdd(&out[OB2 + 16 * t2 + r], gb2[t2]);
         }
-        out[OB3 + r] = gb3;
+        atomicAdd(&out[OB3 + r], gb3);
     }
-    if (lane == 0) lpart[w] = lsum;
+    if (lane == 0) atomicAdd(&sAcc[NPARAM], lsum);
+    __syncthreads();
+    float *dst = part + int64_t(blockIdx.x) * NPARAM;
+    for (int i = threadIdx.x; i < NPARAM; i += blockDim.x) dst[i] = sAcc[i];
+    if (threadIdx.x == 0) lpart[blockIdx.x] = sAcc[NPARAM];
 }
 
-// sum of the per-wave partials; optional Adam step on the float32 master
+// sum of the per-block partials; optional Adam step on the float32 master
 // parameters (Kingma & Ba 2015) and refresh of the bfloat16 weight copy
 __global__ void __launch_bounds__(256) k_reduce_adam(const float *__restrict__ part, int nw, float *__restrict__ grad,
                                                      float *__restrict__ p, float *__restrict__ m,
@@ -221,11 +228,18 @@ __global__ void k_refresh_bf16(const float *__restrict__ p, unsigned short *__re
     if (i < NWB) wb[i] = to_bf16(p[i]);
 }
 
-__global__ void k_sum(const float *__restrict__ a, int n, float *__restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one block of 256 threads: out = Σ a[0..n)
+__global__ void __launch_bounds__(256) k_sum(const float *__restrict__ a, int n, float *__restrict__ out) {
+    __shared__ float red[256];
     float s = 0.f;
-    for (int i = 0; i < n; ++i) s += a[i];
-    *out = s;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += a[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 }  // namespace mlp
@@ -255,7 +269,14 @@ struct Buf {
     }
 };
 
-constexpr int GRID = 64;  // blocks of the gradient kernel (GRID·WAVES partials)
+constexpr int GRID_MAX = 1024;  // blocks of the gradient kernel (one partial gradient each)
+
+// blocks for n points: about two 16-point tiles per wave, at most GRID_MAX
+int gradient_grid(int64_t n) {
+    const int64_t tiles = (n + 15) / 16;
+    const int64_t blocks = (tiles + 2 * WAVES - 1) / (2 * WAVES);
+    return int(std::max<int64_t>(1, std::min<int64_t>(blocks, GRID_MAX)));
+}
 }  // namespace
 
 struct bld_mlp {
@@ -283,8 +304,8 @@ int bld_mlp_create(const float *params, bld_mlp **out) {
     *out = nullptr;
     std::unique_ptr<bld_mlp> h(new bld_mlp());
     if (!h->params.alloc(NPARAM * 4) || !h->m.alloc(NPARAM * 4) || !h->v.alloc(NPARAM * 4) ||
-        !h->wb.alloc(NWB * 2) || !h->grad.alloc(NPARAM * 4) || !h->part.alloc(size_t(GRID) * WAVES * NPARAM * 4) ||
-        !h->lpart.alloc(GRID * WAVES * 4) || !h->loss.alloc(4)) {
+        !h->wb.alloc(NWB * 2) || !h->grad.alloc(NPARAM * 4) || !h->part.alloc(size_t(GRID_MAX) * NPARAM * 4) ||
+        !h->lpart.alloc(GRID_MAX * 4) || !h->loss.alloc(4)) {
         g_mlp_err = "out of device memory";
         return BLD_ENOMEM;
     }
@@ -329,12 +350,15 @@ int bld_mlp_forward(bld_mlp *h, const float *x, int64_t n, float *y) {
     return BLD_OK;
 }
 
-static int run_gradient(bld_mlp *h, const float *X, const float *T, int64_t n) {
+// gradient of the batch into the per-block partials, its loss sum into *loss_dst
+static int run_gradient(bld_mlp *h, const float *X, const float *T, int64_t n, float *loss_dst, int *grid_out) {
     const float scale = 2.0f / float(double(n) * NOUT);
-    k_gradient<<<GRID, 256>>>(h->wb.as<unsigned short>(), h->params.as<float>(), X, T, n, scale, h->part.as<float>(),
+    const int grid = gradient_grid(n);
+    k_gradient<<<grid, 256>>>(h->wb.as<unsigned short>(), h->params.as<float>(), X, T, n, scale, h->part.as<float>(),
                              h->lpart.as<float>());
-    k_sum<<<1, 1>>>(h->lpart.as<float>(), GRID * WAVES, h->loss.as<float>());
+    k_sum<<<1, 256>>>(h->lpart.as<float>(), grid, loss_dst);
     MLP_CHECK(hipGetLastError());
+    *grid_out = grid;
     return BLD_OK;
 }
 
@@ -344,9 +368,10 @@ int bld_mlp_gradient(bld_mlp *h, const float *x, const float *t, int64_t n, floa
     if (!X.alloc(size_t(n) * PAD * 4) || !T.alloc(size_t(n) * PAD * 4)) return BLD_ENOMEM;
     MLP_CHECK(hipMemcpy(X.p, x, size_t(n) * PAD * 4, hipMemcpyHostToDevice));
     MLP_CHECK(hipMemcpy(T.p, t, size_t(n) * PAD * 4, hipMemcpyHostToDevice));
-    int rc = run_gradient(h, X.as<float>(), T.as<float>(), n);
+    int grid = 0;
+    int rc = run_gradient(h, X.as<float>(), T.as<float>(), n, h->loss.as<float>(), &grid);
     if (rc) return rc;
-    k_reduce_adam<<<(NPARAM + 255) / 256, 256>>>(h->part.as<float>(), GRID * WAVES, h->grad.as<float>(), nullptr,
+    k_reduce_adam<<<(NPARAM + 255) / 256, 256>>>(h->part.as<float>(), grid, h->grad.as<float>(), nullptr,
                                                   nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1, 1);
     MLP_CHECK(hipDeviceSynchronize());
     MLP_CHECK(hipMemcpy(grad, h->grad.p, NPARAM * 4, hipMemcpyDeviceToHost));
@@ -368,24 +393,28 @@ int bld_mlp_set_data(bld_mlp *h, const float *x, const float *t, int64_t n) {
 int bld_mlp_train(bld_mlp *h, int64_t steps, int64_t batch, float lr, double *losses) {
     if (!h || steps < 0 || batch < 16 || h->ndata < batch) return BLD_EARG;
     const int64_t nb = h->ndata / batch;
+    Buf L;  // batch losses stay on the device until the end
+    if (!L.alloc(size_t(std::max<int64_t>(steps, 1)) * 4)) return BLD_ENOMEM;
     for (int64_t s = 0; s < steps; ++s) {
         const int64_t off = (h->step % nb) * batch;
-        int rc = run_gradient(h, h->X.as<float>() + off * PAD, h->T.as<float>() + off * PAD, batch);
+        int grid = 0;
+        int rc = run_gradient(h, h->X.as<float>() + off * PAD, h->T.as<float>() + off * PAD, batch,
+                              L.as<float>() + s, &grid);
         if (rc) return rc;
         h->step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float bc1 = 1.f - std::pow(b1, float(h->step)), bc2 = 1.f - std::pow(b2, float(h->step));
-        k_reduce_adam<<<(NPARAM + 255) / 256, 256>>>(h->part.as<float>(), GRID * WAVES, h->grad.as<float>(),
+        k_reduce_adam<<<(NPARAM + 255) / 256, 256>>>(h->part.as<float>(), grid, h->grad.as<float>(),
                                                       h->params.as<float>(), h->m.as<float>(), h->v.as<float>(),
                                                       h->wb.as<unsigned short>(), 1, lr, b1, b2, 1e-8f, bc1, bc2);
         MLP_CHECK(hipGetLastError());
-        if (losses) {
-            float l = 0;
-            MLP_CHECK(hipMemcpy(&l, h->loss.p, 4, hipMemcpyDeviceToHost));
-            losses[s] = double(l) / (double(batch) * NOUT);
-        }
     }
     MLP_CHECK(hipDeviceSynchronize());
+    if (losses && steps > 0) {
+        std::vector<float> l(steps);
+        MLP_CHECK(hipMemcpy(l.data(), L.p, size_t(steps) * 4, hipMemcpyDeviceToHost));
+        for (int64_t s = 0; s < steps; ++s) losses[s] = double(l[s]) / (double(batch) * NOUT);
+    }
     return BLD_OK;
 }
 
@@ -411,8 +440,11 @@ int bld_mlp_time(bld_mlp *h, int64_t n, int reps, double *ms_forward, double *ms
     if (ms_forward) *ms_forward = ms / reps;
     MLP_CHECK(hipEventRecord(a, nullptr));
     for (int r = 0; r < reps; ++r) {
-        int rc = run_gradient(h, X.as<float>(), Y.as<float>(), n);
+        int grid = 0;
+        int rc = run_gradient(h, X.as<float>(), Y.as<float>(), n, h->loss.as<float>(), &grid);
         if (rc) return rc;
+        k_reduce_adam<<<(NPARAM + 255) / 256, 256>>>(h->part.as<float>(), grid, h->grad.as<float>(), nullptr,
+                                                      nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1, 1);
     }
     MLP_CHECK(hipEventRecord(b, nullptr));
     MLP_CHECK(hipEventSynchronize(b));

@@ -29,14 +29,20 @@ __device__ __forceinline__ unsigned short to_bf16(float f) {
 }
 __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float(uint32_t(b) << 16); }
 
-__device__ __forceinline__ float gelu(float z) {
-    const float s = 0.7978845608028654f;  // sqrt(2/π)
-    return 0.5f * z * (1.0f + tanhf(s * (z + 0.044715f * z * z * z)));
+// GELU, tanh form: z/2·(1 + tanh(u)), u = √(2/π)(z + 0.044715 z³).  With
+// 1 + tanh(u) = 2σ(2u), σ the logistic function, one exp and one reciprocal
+// give it (exp overflow for very negative z gives σ = 0, as it should).
+__device__ __forceinline__ float gelu_sigma(float z) {
+    const float s2 = 2.0f * 0.7978845608028654f;  // 2·sqrt(2/π)
+    const float u2 = s2 * (z + 0.044715f * z * z * z);
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
+// d/dz: σ + z·σ(1−σ)·2√(2/π)(1 + 3·0.044715 z²)
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float s = 0.7978845608028654f;
-    const float t = tanhf(s * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * s * (1.0f + 3.0f * 0.044715f * z * z);
+    const float s2 = 2.0f * 0.7978845608028654f;
+    const float sg = gelu_sigma(z);
+    return sg + z * sg * (1.0f - sg) * s2 * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 __device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {

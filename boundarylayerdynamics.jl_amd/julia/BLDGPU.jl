@@ -186,4 +186,22 @@ function rates!(rates, model::BLD.Model, e::Engine)
     rates
 end
 
+"""
+    set_closure!(engine, params, Δ, wall_feature; guard = true)
+
+Replace the stresses of the engine's StaticSmagorinskyModel by the learned
+closure (`params`: the 6288 Float32 network parameters; `wall_feature`: one
+value 1/(1 + d/Δ) per C-layer).  The Smagorinsky model stays the wall model
+and the fallback; `params = nothing` restores it.
+"""
+function set_closure!(e::Engine, params, Δ, wall_feature; guard = true)
+    p = params === nothing ? C_NULL : convert(Vector{Float32}, params)
+    w = convert(Vector{Float64}, wall_feature)
+    rc = GC.@preserve p w ccall((:bld_sgs_closure, libbld), Cint,
+                                (Ptr{Cvoid}, Ptr{Float32}, Cdouble, Ptr{Cdouble}, Cint),
+                                e.handle, p, Δ, w, guard ? 1 : 0)
+    check(rc, e.handle)
+    e
+end
+
 end # module BLDGPU

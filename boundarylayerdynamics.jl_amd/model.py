@@ -550,6 +550,9 @@ class Model:
                 if self._lib.bld_tuning(self._h, k, e, ctypes.byref(b), ctypes.byref(us)) != 0:
                     break
                 out[name].append((b.value, round(us.value, 2)))
+        b, us = ctypes.c_int(), ctypes.c_double()
+        self._check(self._lib.bld_tuning(self._h, 4, 0, ctypes.byref(b), ctypes.byref(us)))
+        out["stage_chunk"] = (b.value, round(us.value, 2))
         return out
 
     def set_profiling(self, on):

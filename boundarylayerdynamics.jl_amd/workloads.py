@@ -72,10 +72,33 @@ def dns_initial(zc, zi, n1, n2, seed=11, uc=1.5, amp=0.05):
     return np.asfortranarray(u), np.asfortranarray(v), np.asfortranarray(w)
 
 
+def spectral_noise(model, seed=0, amp=0.25, k0=3.0):
+    """Seeded random Fourier coefficients with amplitude amp·exp(−|k|/k0)
+    (integer wavenumbers) for every field of `model`, the k1 = 0 column
+    Hermitian in k2 (real fields); added to a state it gives a smooth
+    broadband perturbation."""
+    K1, K2 = model.k1max + 1, 2 * model.k2max + 1
+    k1 = np.arange(K1)[:, None]
+    k2 = np.concatenate([np.arange(model.k2max + 1), np.arange(-model.k2max, 0)])[None, :]
+    scale = amp * np.exp(-np.sqrt(k1 ** 2 + k2 ** 2) / k0)[:, :, None]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = {}
+    for f in model.fields:
+        n = model.nlayers(f)
+        a = (rng.standard_normal((K1, K2, n)) + 1j * rng.standard_normal((K1, K2, n))) * scale
+        km = model.k2max
+        if km > 0:
+            a[0, K2 - km:, :] = np.conj(a[0, 1:km + 1, :][::-1, :])
+        a[0, 0, :] = 0
+        out[f] = np.asfortranarray(a)
+    return out
+
+
 def algorithmic_bytes(cfg_sizes, kind, precision):
-    """Minimal HBM bytes of one launch of each kernel class (inputs read once,
-    outputs written once).  ``cfg_sizes`` holds K1, K2, n3 and the physical
-    sizes of the advection and SGS spaces."""
+    """Minimal HBM bytes of each kernel class per right-hand side over all
+    spaces (stage_project: per stage; inputs read once, outputs written once).
+    ``cfg_sizes`` holds K1, K2, n3 and the physical sizes of the advection and
+    SGS spaces."""
     bc = 8 if precision == "fp32" else 16
     br = bc // 2
     K1, K2, n3 = cfg_sizes["K1"], cfg_sizes["K2"], cfg_sizes["n3"]
@@ -98,7 +121,7 @@ def algorithmic_bytes(cfg_sizes, kind, precision):
     for k, N1, N2 in spaces:
         per_tgt = [(1, nC), (1, nC), (1, nI)] if k == "adv" else [(3, nC), (3, nC), (3, nI)]
         r2c_x2.append(sum(ns * L * N2 * nk1(N1) * bc + 2 * L * nk * bc for ns, L in per_tgt))
-    out["r2c_x2"] = sum(r2c_x2) / max(1, len(r2c_x2))  # per launch (one launch per space)
+    out["r2c_x2"] = sum(r2c_x2)
     for k, N1, N2 in spaces:
         P = N1 * N2
         if k == "adv":
@@ -106,6 +129,8 @@ def algorithmic_bytes(cfg_sizes, kind, precision):
         else:
             out["sgs_viscosity"] = P * br * ((6 * nC + 3 * nI) + (nC + nI))
             out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (nC + nI) + (5 * nC + 3 * nI))
+            # learned closure: 9 terms read, 6 stresses written
+            out["sgs_mlp"] = P * br * ((6 * nC + 3 * nI) + 6 * nC)
     out["stage_project"] = (3 + 4 + 4) / 3 * S  # stage 1: u,du→utmp; stages 2,3: u,utmp,du→out
     out["check_finite"] = S
     return out
