@@ -7,7 +7,8 @@ top of it.  Build the library with ``build.build()``.
 
 from .model import (  # noqa: F401
     Domain, SmoothWall, RoughWall, FreeSlipBoundary, CustomBoundary, SinusoidalMapping,
-    MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, StaticSmagorinskyModel, LearnedSGSModel, SSPRK33,
+    MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, ConstantMean, StaticSmagorinskyModel,
+    LearnedSGSModel, SSPRK33, SSPRK22, Euler, AB2,
     incompressible_flow, closedchannelflow, openchannelflow, Model, Ensemble, evolve, pdsize,
 )
 from . import _lib  # noqa: F401

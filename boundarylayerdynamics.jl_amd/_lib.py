@@ -12,11 +12,12 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbld.so")
 
-BLD_ABI_VERSION = 1
+BLD_ABI_VERSION = 2
 BLD_OK, BLD_EARG, BLD_EUNSUPPORTED, BLD_EHIP, BLD_EDIVERGED, BLD_ENOMEM = 0, -1, -2, -3, -4, -5
 BLD_FP64, BLD_FP32_MIXED = 0, 1
 BLD_BC_DIRICHLET, BLD_BC_NEUMANN = 0, 1
 BLD_WALL_FREESLIP, BLD_WALL_ROUGH = 0, 1
+BLD_SSPRK33, BLD_SSPRK22, BLD_EULER, BLD_AB2 = 0, 1, 2, 3
 
 # kernel classes reported by bld_kernel_times (order of the engine's KernelKind)
 KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "sgs_stress", "r2c_x1",
@@ -30,7 +31,7 @@ EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_siz
            "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error",
            "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
            "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time",
-           "bld_sgs_closure", "bld_closure_sample")
+           "bld_sgs_closure", "bld_closure_sample", "bld_set_method")
 
 
 class BldBC(ctypes.Structure):
@@ -54,6 +55,8 @@ class BldConfig(ctypes.Structure):
         ("sgs_wall", ctypes.c_int32 * 2), ("reserved2", ctypes.c_int32),
         ("sgs_lc", ctypes.POINTER(ctypes.c_double)), ("sgs_li", ctypes.POINTER(ctypes.c_double)),
         ("sgs_factor_strain", ctypes.c_double * 2), ("sgs_factor_stress", ctypes.c_double * 2),
+        ("mean", ctypes.c_int32 * 3), ("reserved3", ctypes.c_int32),
+        ("mean_value", ctypes.c_double * 3), ("mean_weight", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -109,6 +112,7 @@ def load():
         "bld_mlp_train": ([P, i64, i64, ctypes.c_float, D], ctypes.c_int),
         "bld_mlp_time": ([P, i64, ctypes.c_int, D, D], ctypes.c_int),
         "bld_sgs_closure": ([P, F, ctypes.c_double, D, ctypes.c_int], ctypes.c_int),
+        "bld_set_method": ([P, ctypes.c_int], ctypes.c_int),
         "bld_closure_sample": ([P, ctypes.c_int, ctypes.c_double, D, F, F, i64, ctypes.POINTER(i64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

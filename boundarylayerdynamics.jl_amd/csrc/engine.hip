@@ -152,6 +152,7 @@ struct EngineBase {
     virtual void autotune(int reps) = 0;
     virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
     virtual void set_closure(const float *params, double delta, const double *xw, int flags) = 0;
+    virtual void set_method(int method) = 0;
     virtual void sample_closure(int F, double delta, const double *xw, float *X, float *Y, int64_t cap,
                                 int64_t *count) = 0;
 };
@@ -175,6 +176,9 @@ struct Instance {
     int ana = -1;                        // analysis space of the filtered-DNS sampler (created on demand)
     bool closure = false;                // SGS stresses from the learned closure
     DevBuf d_mlp_w, d_mlp_b, d_mlp_xw;
+    bool has_mean = false;               // ConstantMean projection on vel1/vel2
+    CMeanDesc<T> cmean{};
+    DevBuf d_mean_w;
     LinDesc<T> lin{};
     ProjDesc<T> proj{};
     ProjDesc<double> proj00{};
@@ -213,6 +217,8 @@ static void check_config(const bld_config &c) {
             throw BldError{BLD_EUNSUPPORTED, "Advection for non-zero w at boundary not implemented"};
         if (c.adv_n1 < 1 || c.adv_n2 < 1) throw BldError{BLD_EARG, "invalid advection grid"};
     }
+    if (c.mean[2]) throw BldError{BLD_EUNSUPPORTED, "The constant-mean source currently does not handle I-nodes correctly"};
+    if ((c.mean[0] || c.mean[1]) && !c.mean_weight) throw BldError{BLD_EARG, "missing constant-mean weights"};
     if (c.sgs) {
         if (!w_dir) throw BldError{BLD_EUNSUPPORTED, "the SGS model requires Dirichlet boundary conditions for vel3"};
         if (!c.sgs_lc || !c.sgs_li) throw BldError{BLD_EARG, "missing SGS length scales"};
@@ -284,7 +290,10 @@ class Engine : public EngineBase {
     Pass pp[5];  // physics: advection products, SGS viscosity, SGS stress, learned stress, its rates
     Pass lin_pass, stage_pass, stage00_pass, xfer[4];
     int stage_CL = 4;
-    int stage_CL_tuned = 0;  // chunk length chosen by the autotuner (0: by n3)
+    int stage_CL_tuned = 0;
+    int method = BLD_SSPRK33;
+    bool ab2_started = false;
+    double ab2_dt = 0;  // chunk length chosen by the autotuner (0: by n3)
     // tuning results: per transform pass and entry
     std::vector<double> tuned_us[5];
     // graph
@@ -450,6 +459,22 @@ class Engine : public EngineBase {
         D.flag = d_flag.as<int>();
         D.ext_stride = 0;
         D.skip0 = I.mixed ? 1 : 0;
+        I.has_mean = I.cfg.mean[0] || I.cfg.mean[1];
+        if (I.has_mean) {
+            upload(I.d_mean_w, std::vector<double>(I.cfg.mean_weight, I.cfg.mean_weight + I.n3));
+            CMeanDesc<T> &M = I.cmean;
+            M.n3 = I.n3;
+            M.nk = I.nk;
+            M.w = I.d_mean_w.template as<double>();
+            for (int f = 0; f < 2; ++f) {
+                M.on[f] = I.cfg.mean[f] ? 1 : 0;
+                M.value[f] = I.cfg.mean_value[f];
+                M.u[f] = D.u[f];
+                M.ut[f] = D.ut[f];
+                M.u00[f] = D.u00[f];
+                M.ut00[f] = D.ut00[f];
+            }
+        }
         if (!I.mixed) return;
         ProjDesc<double> &Z = I.proj00;  // one column (k1 = k2 = 0) over the float64 shadows
         Z.K1 = Z.K2 = Z.nk = 1;
@@ -873,6 +898,8 @@ class Engine : public EngineBase {
                                     st);
             if (stage00_pass.nblocks) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
             for (auto &I : inst)
+                if (I->has_mean) launch_constant_mean<T>(I->cmean, s, st);
+            for (auto &I : inst)
                 if (I->sequential_stage) launch_stage_sequential<T>(I->proj, s, dt, st);
         });
     }
@@ -884,13 +911,57 @@ class Engine : public EngineBase {
         });
     }
 
+    // perform_step! of the selected method followed by rate! (ODEMethods.jl:100-166,231);
+    // the Adams–Bashforth start step is taken outside graphs (ab2_start)
     void one_step(double dt) {
-        stage(1, dt);
-        rhs(1);
-        stage(2, dt);
-        rhs(1);
-        stage(3, dt);
+        switch (method) {
+            case BLD_EULER:
+                stage(STAGE_EULER, dt);
+                rhs(0);
+                break;
+            case BLD_SSPRK22:
+                stage(STAGE_RK_1, dt);
+                rhs(1);
+                stage(STAGE_RK2_2, dt);
+                rhs(0);
+                break;
+            case BLD_AB2:
+                stage(STAGE_AB2, dt);
+                rhs(0);
+                break;
+            default:
+                stage(STAGE_RK_1, dt);
+                rhs(1);
+                stage(STAGE_RK3_2, dt);
+                rhs(1);
+                stage(STAGE_RK3_3, dt);
+                rhs(0);
+        }
+    }
+
+    // the first Adams–Bashforth step is a two-stage Runge-Kutta step that also
+    // keeps f(u0) (ODEMethods.jl:106-124); later steps need the same dt
+    int64_t ab2_start(double dt, int64_t n) {
+        if (method != BLD_AB2 || n == 0) return n;
+        if (ab2_started) {
+            if (dt != ab2_dt) throw BldError{BLD_EARG, "Cannot change time step for multi-step methods"};
+            return n;
+        }
+        stage(STAGE_AB2_FIRST_1, dt);
         rhs(0);
+        stage(STAGE_AB2_FIRST_2, dt);
+        rhs(0);
+        ab2_started = true;
+        ab2_dt = dt;
+        return n - 1;
+    }
+
+    void set_method(int m) override {
+        if (m != BLD_SSPRK33 && m != BLD_SSPRK22 && m != BLD_EULER && m != BLD_AB2)
+            throw BldError{BLD_EARG, "unknown ODE method"};
+        if (m != method) drop_graph();
+        method = m;
+        ab2_started = false;
     }
 
     int read_flag() {
@@ -965,6 +1036,7 @@ class Engine : public EngineBase {
 
     int init_problem() override {
         clear_flag();
+        ab2_started = false;
         stage(0, 0.0);
         rhs(0);
         check_du();
@@ -991,7 +1063,8 @@ class Engine : public EngineBase {
 
     int step(double dt, int64_t n, bool graph) override {
         clear_flag();
-        if (graph) capture(dt);
+        n = ab2_start(dt, n);
+        if (graph && n > 0) capture(dt);
         for (int64_t i = 0; i < n; ++i) {
             if (graph)
                 HIPCHECK(hipGraphLaunch(graph_exec, st));
@@ -1004,6 +1077,7 @@ class Engine : public EngineBase {
     }
 
     double time_steps(double dt, int64_t n, bool graph) override {
+        n = ab2_start(dt, n);
         if (graph) capture(dt);
         hipEvent_t a, b;
         HIPCHECK(hipEventCreate(&a));
@@ -1487,6 +1561,13 @@ int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wa
     if (!count) return BLD_EARG;
     ENGINE_CALL({
         e->impl->sample_closure(filter, delta, wall_feature, x, t, capacity, count);
+        return BLD_OK;
+    })
+}
+
+int bld_set_method(bld_engine *e, int method) {
+    ENGINE_CALL({
+        e->impl->set_method(method);
         return BLD_OK;
     })
 }

@@ -593,11 +593,19 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
 template <typename T, typename A>
 __device__ __forceinline__ cplx<A> combine(int stage, cplx<A> u, cplx<A> ut, cplx<A> du, A dt) {
     switch (stage) {
-        case 1: return mk(u.x + dt * du.x, u.y + dt * du.y);  // utmp = u + dt·du
-        case 2:
+        case STAGE_RK_1: return mk(u.x + dt * du.x, u.y + dt * du.y);  // utmp = u + dt·du
+        case STAGE_RK3_2:
             return mk((A(3) * u.x + ut.x + dt * du.x) / A(4), (A(3) * u.y + ut.y + dt * du.y) / A(4));
-        case 3:
+        case STAGE_RK3_3:
             return mk((u.x + A(2) * ut.x + A(2) * dt * du.x) / A(3), (u.y + A(2) * ut.y + A(2) * dt * du.y) / A(3));
+        case STAGE_EULER:
+        case STAGE_AB2_FIRST_1: return mk(u.x + dt * du.x, u.y + dt * du.y);  // u += dt·du
+        case STAGE_RK2_2:  // u = (u + utmp + dt·du)/2
+            return mk((u.x + ut.x + dt * du.x) / A(2), (u.y + ut.y + dt * du.y) / A(2));
+        case STAGE_AB2:  // u += dt/2·(3·du − du_prev), du_prev in utmp
+            return mk(u.x + dt / A(2) * (A(3) * du.x - ut.x), u.y + dt / A(2) * (A(3) * du.y - ut.y));
+        case STAGE_AB2_FIRST_2:  // u += dt/2·(du − du_prev)
+            return mk(u.x + dt / A(2) * (du.x - ut.x), u.y + dt / A(2) * (du.y - ut.y));
         default: return u;  // projection of the current state only
     }
 }
@@ -608,9 +616,10 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
     if (m >= D.nk || m >= mode_limit) return;
     const int nk = D.nk, n3 = D.n3;
     const bool mixed00 = (m == 0) && (D.u00[0] != nullptr);
-    // input/output selection: stage 0 projects u in place, 1-2 write utmp, 3 writes u
-    cplx<T> *const *outp = (stage == 1 || stage == 2) ? D.ut : D.u;
-    cplx<double> **outs = (stage == 1 || stage == 2) ? D.ut00 : D.u00;
+    // input/output selection: stage 0 projects u in place, the first stages of
+    // the Runge-Kutta methods write utmp, the others u
+    cplx<T> *const *outp = stage_writes_ut(stage) ? D.ut : D.u;
+    cplx<double> **outs = stage_writes_ut(stage) ? D.ut00 : D.u00;
     bool bad = false;
     auto fin = [&](cplx<T> v) {
         if (!(isfinite(v.x) && isfinite(v.y))) bad = true;
@@ -625,16 +634,19 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
                 du = mk(D.du00[f][l].x + double(dn.x), D.du00[f][l].y + double(dn.y));
                 if (!(isfinite(du.x) && isfinite(du.y))) bad = true;
             }
-            return combine<T, double>(stage, D.u00[f][l], D.ut00[f][l], du, dt);
+            const cplx<double> c = combine<T, double>(stage, D.u00[f][l], D.ut00[f][l], du, dt);
+            if (stage_saves_du(stage)) D.ut00[f][l] = du;
+            return c;
         }
         const cplx<T> u = D.u[f][i];
         cplx<T> ut = mk(T(0), T(0)), du = mk(T(0), T(0));
-        if (stage == 2 || stage == 3) ut = D.ut[f][i];
+        if (stage_reads_ut(stage)) ut = D.ut[f][i];
         if (stage > 0) {
             du = D.du[f][i];
             fin(du);
         }
         const cplx<T> c = combine<T, T>(stage, u, ut, du, T(dt));
+        if (stage_saves_du(stage)) D.ut[f][i] = du;
         return mk(double(c.x), double(c.y));
     };
     auto store = [&](int f, int l, cplx<double> v) {
@@ -750,7 +762,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     const bool active = m < nk && !(skip0 && m == 0);
     const int l0 = ty * CL;
     const int mm = active ? m : 0;
-    cplx<T> *const *outp = (stage == 1 || stage == 2) ? D.ut : D.u;
+    cplx<T> *const *outp = stage_writes_ut(stage) ? D.ut : D.u;
     bool bad = false;
     cplx<T> c1[CL], c2[CL], c3[CL];
 #pragma unroll
@@ -762,7 +774,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
                 if (f == 2 && l == n3 - 1) break;
                 const size_t i = size_t(l) * nk + m;
                 cplx<T> u = D.u[f][i], ut = mk(T(0), T(0)), du = mk(T(0), T(0));
-                if (stage == 2 || stage == 3) ut = D.ut[f][i];
+                if (stage_reads_ut(stage)) ut = D.ut[f][i];
                 if (stage > 0) {
                     du = D.du[f][i];
                     if (D.ext_stride) {
@@ -772,6 +784,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
                     if (!(isfinite(du.x) && isfinite(du.y))) bad = true;
                 }
                 const cplx<T> c = combine<T, T>(stage, u, ut, du, T(dt));
+                if (stage_saves_du(stage)) D.ut[f][i] = du;  // Adams–Bashforth: keep f(u) for the next step
                 if (f == 0) c1[j] = c;
                 else if (f == 1) c2[j] = c;
                 else c3[j] = c;
@@ -790,7 +803,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
                 outp[1][i] = c2[j];
                 if (l < n3 - 1) outp[2][i] = c3[j];
                 if (D.ext_stride) {
-                    cplx<float> *const *mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+                    cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
                     const size_t e = size_t(l) * D.ext_stride;
                     mir[0][e] = mk(float(c1[j].x), float(c1[j].y));
                     mir[1][e] = mk(float(c2[j].x), float(c2[j].y));
@@ -915,7 +928,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
         v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
         outp[0][i] = mk(T(v1.x), T(v1.y));
         outp[1][i] = mk(T(v2.x), T(v2.y));
-        cplx<float> *const *mir = (stage == 1 || stage == 2) ? D.mirror_ut : D.mirror_u;
+        cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
         const size_t e = size_t(l) * D.ext_stride;
         if (D.ext_stride) {
             mir[0][e] = mk(float(v1.x), float(v1.y));
@@ -929,6 +942,38 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
             v3 = mk(v3.x - gr.x, v3.y - gr.y);
             outp[2][i] = mk(T(v3.x), T(v3.y));
             if (D.ext_stride) mir[2][e] = mk(float(v3.x), float(v3.y));
+        }
+    }
+}
+
+// ConstantMean projection (sources.jl:63-77): shift the (0,0) column of each
+// field with a prescribed mean so that Σ_l w_l·Re q(0,0,l) equals it; one block
+// per instance, the float64 shadow first where the mixed form keeps one
+template <typename T>
+__global__ void __launch_bounds__(256) k_constant_mean(CMeanDesc<T> D, int out_ut) {
+    __shared__ double red[256];
+    for (int f = 0; f < 2; ++f) {
+        if (!D.on[f]) continue;
+        cplx<T> *q = out_ut ? D.ut[f] : D.u[f];
+        cplx<double> *q00 = out_ut ? D.ut00[f] : D.u00[f];
+        double s = 0.0;
+        for (int l = threadIdx.x; l < D.n3; l += blockDim.x)
+            s += D.w[l] * (q00 ? q00[l].x : double(q[size_t(l) * D.nk].x));
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if (int(threadIdx.x) < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        const double shift = D.value[f] - red[0];
+        __syncthreads();
+        for (int l = threadIdx.x; l < D.n3; l += blockDim.x) {
+            if (q00) {
+                q00[l].x += shift;
+                q[size_t(l) * D.nk].x = T(q00[l].x);
+            } else {
+                q[size_t(l) * D.nk].x = T(double(q[size_t(l) * D.nk].x) + shift);
+            }
         }
     }
 }
@@ -971,6 +1016,11 @@ void launch_physics(int kind, const Space<T> *spaces_dev, const LaunchTable &lt,
         case PHYS_SGS_VISC: k_sgs_viscosity<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
         case PHYS_SGS_STRESS: k_sgs_stress<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
     }
+}
+
+template <typename T>
+void launch_constant_mean(const CMeanDesc<T> &D, int stage, hipStream_t st) {
+    k_constant_mean<T><<<1, 256, 0, st>>>(D, stage_writes_ut(stage) ? 1 : 0);
 }
 
 template <typename T>
@@ -1017,6 +1067,7 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
     template void launch_transform<T>(int, const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
     template void launch_physics<T>(int, const Space<T> *, const LaunchTable &, int, hipStream_t);                \
     template void launch_linear<T>(const LinDesc<T> *, const LaunchTable &, int, int, hipStream_t);               \
+    template void launch_constant_mean<T>(const CMeanDesc<T> &, int, hipStream_t);                                \
     template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t); \
     template void launch_stage_sequential<T>(const ProjDesc<T> &, int, double, hipStream_t);                      \
     template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \

@@ -49,6 +49,37 @@ struct ProjDesc {
     cplx<float> *mirror_u[3], *mirror_ut[3];
 };
 
+// Stage kinds of the fused stage kernels (ODEMethods.jl:100-166): the
+// Runge-Kutta stages, forward Euler and the two-step Adams–Bashforth update
+// (with its Runge-Kutta start), each followed by the projection.
+enum StageKind : int {
+    STAGE_PROJECT = 0,      // projection of u only
+    STAGE_RK_1 = 1,         // utmp = u + dt·du (SSPRK33 and SSPRK22 stage 1)
+    STAGE_RK3_2 = 2,        // utmp = (3u + utmp + dt·du)/4
+    STAGE_RK3_3 = 3,        // u = (u + 2·utmp + 2dt·du)/3
+    STAGE_EULER = 4,        // u = u + dt·du
+    STAGE_RK2_2 = 5,        // u = (u + utmp + dt·du)/2
+    STAGE_AB2 = 6,          // u = u + dt/2·(3du − utmp); utmp = du
+    STAGE_AB2_FIRST_1 = 7,  // u = u + dt·du; utmp = du
+    STAGE_AB2_FIRST_2 = 8,  // u = u + dt/2·(du − utmp)
+};
+__host__ __device__ constexpr bool stage_writes_ut(int s) { return s == STAGE_RK_1 || s == STAGE_RK3_2; }
+__host__ __device__ constexpr bool stage_reads_ut(int s) {
+    return s == STAGE_RK3_2 || s == STAGE_RK3_3 || s == STAGE_RK2_2 || s == STAGE_AB2 || s == STAGE_AB2_FIRST_2;
+}
+__host__ __device__ constexpr bool stage_saves_du(int s) { return s == STAGE_AB2 || s == STAGE_AB2_FIRST_1; }
+
+// ConstantMean projection of one model (vel1, vel2 at C-nodes)
+template <typename T>
+struct CMeanDesc {
+    int on[2];
+    int n3, nk;
+    double value[2];
+    const double *w;  // [n3] weights 1/(α_c·L3)
+    cplx<T> *u[2], *ut[2];
+    cplx<double> *u00[2], *ut00[2];  // float64 shadows of the mixed form (else null)
+};
+
 template <typename T>
 void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel,
                       hipStream_t st);
@@ -61,6 +92,8 @@ void launch_closure(int kind, const Space<T> *spaces_dev, const LaunchTable &lt,
 template <typename T>
 void launch_closure_sample(const Space<T> *space_dev, int N1, int N2, int n3, int F, double delta2, const double *xw,
                            float *X, float *Y, hipStream_t st);
+template <typename T>
+void launch_constant_mean(const CMeanDesc<T> &D, int stage, hipStream_t st);
 template <typename T>
 void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st);
 template <typename T>

@@ -15,7 +15,7 @@ module BLDGPU
 using BoundaryLayerDynamics: BoundaryLayerDynamics as BLD
 
 const libbld = get(ENV, "BLD_LIBRARY", joinpath(@__DIR__, "..", "libbld.so"))
-const ABI_VERSION = Int32(1)
+const ABI_VERSION = Int32(2)
 
 struct BldBC
     kind::Int32       # 0 Dirichlet, 1 Neumann
@@ -53,6 +53,10 @@ struct BldConfig
     sgs_li::Ptr{Float64}
     sgs_factor_strain::NTuple{2,Float64}
     sgs_factor_stress::NTuple{2,Float64}
+    mean::NTuple{3,Int32}
+    reserved3::Int32
+    mean_value::NTuple{3,Float64}
+    mean_weight::Ptr{Float64}
 end
 
 mutable struct Engine
@@ -92,6 +96,7 @@ function Engine(model::BLD.Model; precision = :fp64)
     adv_n = sgs_n = (Int32(0), Int32(0))
     lc = li = Float64[]
     walls, fstrain, fstress = (Int32(0), Int32(0)), (0.0, 0.0), (0.0, 0.0)
+    mean, mean_value, mean_weight = zeros(Int32, 3), zeros(3), Float64[]
     fidx = Dict(:vel1 => 1, :vel2 => 2, :vel3 => 3)
     P = BLD.Processes
     for p in model.processes
@@ -104,6 +109,10 @@ function Engine(model::BLD.Model; precision = :fp64)
             pressure = Int32(1)
         elseif p isa P.DiscretizedConstantSource
             source[fidx[p.field]] += p.strength
+        elseif p isa P.DiscretizedConstantMean
+            mean[fidx[p.field]] = 1
+            mean_value[fidx[p.field]] = p.mean_value
+            mean_weight = collect(Float64, p.weight)
         elseif p isa P.DiscretizedStaticSmagorinskyModel
             sgs, sgs_n = Int32(1), Int32.(p.dims)
             lc, li = vec(p.lengthscale_c), vec(p.lengthscale_i)
@@ -114,13 +123,14 @@ function Engine(model::BLD.Model; precision = :fp64)
             error("process $(nameof(p)) is not part of the GPU step engine")
         end
     end
-    push!(keep, lc, li)
+    push!(keep, lc, li, mean_weight)
     bcs = (bcpair(domain, grid, :vel1)..., bcpair(domain, grid, :vel2)..., bcpair(domain, grid, :vel3)...)
     cfg = BldConfig(ABI_VERSION, precision == :fp32 ? 1 : 0, grid.k1max, grid.k2max, n3, 0,
                     domain.hsize[1], domain.hsize[2], pointer(αc), pointer(αi), bcs,
                     Tuple(diffusion), 0, Tuple(diffusivity), Tuple(source), pressure, advection, adv_n...,
                     sgs, sgs_n..., walls, 0, isempty(lc) ? C_NULL : pointer(lc),
-                    isempty(li) ? C_NULL : pointer(li), fstrain, fstress)
+                    isempty(li) ? C_NULL : pointer(li), fstrain, fstress,
+                    Tuple(mean), 0, Tuple(mean_value), isempty(mean_weight) ? C_NULL : pointer(mean_weight))
     h = Ref{Ptr{Cvoid}}(C_NULL)
     rc = GC.@preserve keep ccall((:bld_create, libbld), Cint, (Ref{BldConfig}, Ref{Ptr{Cvoid}}), cfg, h)
     check(rc, C_NULL)

@@ -132,6 +132,15 @@ class ConstantSource:
         self.strength = strength
 
 
+class ConstantMean:
+    """sources.jl:47-77: a projection that keeps the z-weighted mean of the
+    (0,0) mode of `field` at `mean_value` (constant flux)."""
+
+    def __init__(self, field, mean_value=1.0):
+        self.field = field
+        self.mean_value = mean_value
+
+
 class StaticSmagorinskyModel:
     def __init__(self, dealiasing=None, Cs=0.1, wall_damping=True, wall_damping_exponent=2):
         self.dealiasing = dealiasing
@@ -179,18 +188,43 @@ def wall_feature(domain, n3, delta):
 
 
 class SSPRK33:
-    """Three-stage SSP Runge-Kutta (ODEMethods.jl:50,147-166), the engine's method."""
+    """Three-stage SSP Runge-Kutta (ODEMethods.jl:50,147-166), the default method."""
+
+    code = _lib.BLD_SSPRK33
+
+
+class SSPRK22:
+    """Two-stage SSP Runge-Kutta (ODEMethods.jl:49,133-145)."""
+
+    code = _lib.BLD_SSPRK22
+
+
+class Euler:
+    """Forward Euler (ODEMethods.jl:47,100-103)."""
+
+    code = _lib.BLD_EULER
+
+
+class AB2:
+    """Two-step Adams–Bashforth with a Runge-Kutta first step
+    (ODEMethods.jl:48,105-131); dt must stay constant."""
+
+    code = _lib.BLD_AB2
 
 
 def incompressible_flow(viscosity, sgs_model=None, constant_forcing=None, constant_flux=None):
     """BoundaryLayerDynamics.jl:352-368."""
-    if constant_flux is not None:
-        raise NotImplementedError("ConstantMean (constant flux) is not part of the GPU step engine")
+    if constant_flux is not None and constant_forcing is not None:
+        raise ValueError("Momentum source set up for both constant forcing and constant flux")
     procs = [MomentumAdvection()] + [MolecularDiffusion(v, viscosity) for v in ("vel1", "vel2", "vel3")]
     procs.append(Pressure())
-    forcing = 1 if constant_forcing is None else constant_forcing
-    forcing = forcing if isinstance(forcing, (tuple, list)) else (forcing,)
-    procs += [ConstantSource(v, f) for v, f in zip(("vel1", "vel2"), forcing)]
+    if constant_flux is None:
+        forcing = 1 if constant_forcing is None else constant_forcing
+        forcing = forcing if isinstance(forcing, (tuple, list)) else (forcing,)
+        procs += [ConstantSource(v, f) for v, f in zip(("vel1", "vel2"), forcing)]
+    else:
+        flux = constant_flux if isinstance(constant_flux, (tuple, list)) else (constant_flux,)
+        procs += [ConstantMean(v, m) for v, m in zip(("vel1", "vel2"), flux)]
     if sgs_model is not None:
         procs.append(sgs_model)
     return procs
@@ -336,6 +370,13 @@ class Discretization:
                 c.pressure = 1
             elif isinstance(p, ConstantSource):
                 c.source[FIELD_INDEX[p.field]] += float(p.strength)
+            elif isinstance(p, ConstantMean):
+                if NODES[p.field] != "C":
+                    raise NotImplementedError("The constant-mean source currently does not handle I-nodes correctly")
+                fi = FIELD_INDEX[p.field]
+                c.mean[fi] = 1
+                c.mean_value[fi] = float(p.mean_value)
+                c.mean_weight = self._arr([1.0 / alpha(z) / d.size(3) for z in _zeta(n3, "C")])
             elif isinstance(p, StaticSmagorinskyModel):
                 if c.sgs:
                     raise NotImplementedError("more than one SGS process")
@@ -525,6 +566,14 @@ class Model:
     def init_problem(self, t1=0.0):
         self._check(self._lib.bld_init_problem(self._h, t1))
 
+    def set_method(self, method):
+        """ODE method of the step calls (an instance or class of SSPRK33,
+        SSPRK22, Euler, AB2); applies to every model of an ensemble."""
+        code = getattr(method, "code", None)
+        if code is None:
+            raise NotImplementedError(f"ODE method {method!r} is not part of the GPU step engine")
+        self._check(self._lib.bld_set_method(self._h, code))
+
     def step(self, dt, nsteps, t=0.0, t_end=0.0, graph=True):
         tt = ctypes.c_double(t)
         fn = self._lib.bld_step_graph if graph else self._lib.bld_step
@@ -611,10 +660,10 @@ def evolve(model, tspan, dt, method=None, graph=True):
     """evolve! (BoundaryLayerDynamics.jl:190-219) without output modules.
 
     The state is projected and its rate evaluated at t1 (ODEProblem,
-    ODEMethods.jl:179-185), then nt SSPRK33 steps run on the GPU, each followed
-    by the rate evaluation of ODEMethods.jl:231.  Returns the final time."""
-    if method is not None and not isinstance(method, SSPRK33):
-        raise NotImplementedError("the GPU engine integrates with SSPRK33")
+    ODEMethods.jl:179-185), then nt steps of `method` (default SSPRK33) run on
+    the GPU, each followed by the rate evaluation of ODEMethods.jl:231.
+    Returns the final time."""
+    model.set_method(SSPRK33() if method is None else method)
     t1, t2 = (0.0, float(tspan)) if np.isscalar(tspan) else (float(tspan[0]), float(tspan[-1]))
     if not t1 <= t2:
         raise ValueError("Integration start time is larger than end time")
@@ -663,6 +712,8 @@ class Ensemble:
         t = ctypes.c_double(0.0)
         fn = self._lib.bld_step_graph if graph else self._lib.bld_step
         self._check(fn(self._h, dt, nsteps, ctypes.byref(t), 0.0))
+
+    set_method = Model.set_method
 
     def time_steps(self, dt, nsteps, graph=True):
         ms = ctypes.c_double()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BLD_ABI_VERSION 1
+#define BLD_ABI_VERSION 2
 
 typedef enum bld_status {
     BLD_OK = 0,
@@ -88,6 +88,12 @@ typedef struct bld_config {
     const double *sgs_li;   /* [n3-1] same at interior I-nodes */
     double sgs_factor_strain[2];  /* factors_strain (:92-108) */
     double sgs_factor_stress[2];  /* factors_stress */
+    /* ConstantMean (sources.jl:47-77), vel1/vel2 only: projection that shifts
+     * the (0,0) mode so that Σ_l weight[l]·Re q(0,0,l) = mean_value */
+    int32_t mean[3];
+    int32_t reserved3;
+    double mean_value[3];
+    const double *mean_weight;  /* [n3] 1/(α_c·L3) (sources.jl:63-66) */
 } bld_config;
 
 typedef struct bld_engine bld_engine;
@@ -141,6 +147,13 @@ int bld_sgs_closure(bld_engine *e, const float *params, double delta, const doub
  * only *count is set. */
 int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wall_feature, float *x, float *t,
                        int64_t capacity, int64_t *count);
+
+/* Time-integration method of the step calls (ODEMethods.jl:12-166); the
+ * default is SSPRK33.  AB2 takes its first step as the reference does (a
+ * two-stage Runge-Kutta step that keeps f(u0)) and then requires a constant
+ * dt; bld_init_problem restarts it. */
+typedef enum bld_method { BLD_SSPRK33 = 0, BLD_SSPRK22 = 1, BLD_EULER = 2, BLD_AB2 = 3 } bld_method;
+int bld_set_method(bld_engine *e, int method);
 
 /* Message of the last failed call on `e` (or of the last failed bld_create
  * on this thread when e is NULL). */

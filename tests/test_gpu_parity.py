@@ -47,6 +47,9 @@ CASES = {
     # closed channel DNS, smooth walls (C1-like, small)
     "channel": dict(res=(12, 10, 8), dims=(2 * np.pi, np.pi, 2.0), lower="SmoothWall", upper="SmoothWall",
                     procs=lambda M: M.incompressible_flow(1 / 180, constant_forcing=1)),
+    # closed channel with constant flux (ConstantMean projection, sources.jl:47-77)
+    "channel_flux": dict(res=(12, 10, 8), dims=(2 * np.pi, np.pi, 2.0), lower="SmoothWall", upper="SmoothWall",
+                         mapping=0.5, procs=lambda M: M.incompressible_flow(1 / 180, constant_flux=(1.0, 0.2))),
     # neutral half channel LES, rough wall + free slip (C2-like, small)
     "les": dict(res=(16, 16, 8), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4), upper="FreeSlipBoundary",
                 procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
@@ -129,7 +132,7 @@ def test_rates_nonhermitian_input(name):
         assert relerr(got[f], rates[f]) < RTOL_RHS64, (name, f)
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "tiny"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "tiny", "channel_flux"])
 def test_projection_fp64(name):
     ref, gpu = pair(name, seed=5)
     O.apply_projections(ref, ref.state)
@@ -140,6 +143,7 @@ def test_projection_fp64(name):
 
 
 @pytest.mark.parametrize("name,steps,dt", [("channel", 5, 1e-3), ("les", 5, 5e-4), ("stretched", 4, 1e-3),
+                                           ("channel_flux", 5, 1e-3),
                                            ("les_dealiased", 3, 1e-3), ("tiny", 6, 1e-2), ("custom", 4, 1e-3)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_evolve_fp64(name, steps, dt, graph):
@@ -377,3 +381,47 @@ def test_autotune_keeps_results(name):
     got = gpu.state
     for f in ref.fields:
         assert relerr(got[f], ref.state[f]) < RTOL64
+
+
+@pytest.mark.parametrize("method", ["Euler", "SSPRK22", "AB2"])
+@pytest.mark.parametrize("name,dt", [("channel", 1e-3), ("les", 5e-4), ("channel_flux", 1e-3)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_other_methods_fp64(method, name, dt, graph):
+    """perform_step! of Euler, SSPRK22 and AB2 (ODEMethods.jl:100-145) against
+    the oracle's, 5 steps (AB2: the Runge-Kutta start and 4 regular steps)."""
+    ref, gpu = pair(name, seed=9)
+    O.evolve(ref, 5 * dt, dt, method=getattr(O, method)())
+    B.evolve(gpu, 5 * dt, dt, method=getattr(B, method)(), graph=graph)
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < RTOL64, (method, name, f, relerr(got[f], ref.state[f]))
+
+
+@pytest.mark.parametrize("method", ["SSPRK22", "AB2"])
+def test_other_methods_fp32(method):
+    ref, gpu = pair("les", precision="fp32", seed=9)
+    O.evolve(ref, 4 * 5e-4, 5e-4, method=getattr(O, method)())
+    B.evolve(gpu, 4 * 5e-4, 5e-4, method=getattr(B, method)())
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < 1e-4, (method, f, relerr(got[f], ref.state[f]))
+
+
+def test_ab2_requires_constant_dt():
+    ref, gpu = pair("channel")
+    gpu.set_method(B.AB2())
+    gpu.init_problem()
+    gpu.step(1e-3, 2)
+    with pytest.raises(B._lib.BldError, match="Cannot change time step"):
+        gpu.step(2e-3, 1)
+
+
+def test_constant_flux_keeps_the_mean():
+    """After every projection the z-weighted mean of the (0,0) mode of vel1 and
+    vel2 equals the prescribed flux (sources.jl:71-77)."""
+    ref, gpu = pair("channel_flux", seed=4)
+    B.evolve(gpu, 0.01, 1e-3)
+    st = gpu.state
+    w = [pd for k, pd in ref.disc if k == "mean"][0]["weight"]
+    assert abs(np.sum(st["vel1"][0, 0, :].real * w) - 1.0) < 1e-12
+    assert abs(np.sum(st["vel2"][0, 0, :].real * w) - 0.2) < 1e-12

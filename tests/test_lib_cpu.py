@@ -70,10 +70,26 @@ def test_sgs_rejects_smooth_wall():
         B.model.Discretization((8, 8, 4), dom, [B.StaticSmagorinskyModel()])
 
 
+def test_constant_mean_constants_match_oracle():
+    mk = lambda M: M.Domain((1.0, 1.0, 2.0), M.SmoothWall(), M.SmoothWall(), M.SinusoidalMapping(0.7))
+    d = B.model.Discretization((8, 8, 6), mk(B), B.incompressible_flow(1e-2, constant_flux=(1.5, 0.25)))
+    ref = O.Model((8, 8, 6), mk(O), O.incompressible_flow(1e-2, constant_flux=(1.5, 0.25)))
+    c = d.config
+    means = [pd for k, pd in ref.disc if k == "mean"]
+    assert list(c.mean) == [1, 1, 0]
+    assert list(c.mean_value)[:2] == [1.5, 0.25]
+    assert np.allclose(np.ctypeslib.as_array(c.mean_weight, (6,)), means[0]["weight"], rtol=1e-15)
+    assert list(c.source) == [0.0, 0.0, 0.0]
+    with pytest.raises(ValueError):
+        B.incompressible_flow(1.0, constant_flux=1.0, constant_forcing=1.0)
+    with pytest.raises(NotImplementedError):
+        B.model.Discretization((8, 8, 6), mk(B), [B.ConstantMean("vel3", 1.0)])
+
+
 def test_unsupported_processes_rejected_on_host():
     dom = B.Domain((1, 1, 1), B.SmoothWall(), B.SmoothWall())
     with pytest.raises(NotImplementedError):
-        B.incompressible_flow(1.0, constant_flux=1.0)
+        B.model.Model.set_method(None, object())
     with pytest.raises(NotImplementedError):
         B.model.Discretization((4, 4, 4), dom, [B.MolecularDiffusion("theta", 1.0)])
     with pytest.raises(ValueError):
