@@ -9,9 +9,10 @@ from .model import (  # noqa: F401
     Domain, SmoothWall, RoughWall, FreeSlipBoundary, CustomBoundary, SinusoidalMapping,
     MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, ConstantMean, StaticSmagorinskyModel,
     LearnedSGSModel, SSPRK33, SSPRK22, Euler, AB2,
-    incompressible_flow, closedchannelflow, openchannelflow, Model, Ensemble, evolve, pdsize,
+    incompressible_flow, closedchannelflow, openchannelflow, Model, Ensemble, evolve, pdsize, Snapshots,
 )
 from . import _lib  # noqa: F401
 from .ldlt import BatchLDLt  # noqa: F401
 from . import workloads  # noqa: F401
 from . import closure  # noqa: F401
+from . import cbd  # noqa: F401

@@ -91,8 +91,8 @@ def load():
         "bld_init_problem": ([P, ctypes.c_double], ctypes.c_int),
         "bld_step": ([P, ctypes.c_double, i64, D, ctypes.c_double], ctypes.c_int),
         "bld_step_graph": ([P, ctypes.c_double, i64, D, ctypes.c_double], ctypes.c_int),
-        "bld_set_field_physical": ([P, ctypes.c_int, D], ctypes.c_int),
-        "bld_get_field_physical": ([P, ctypes.c_int, D], ctypes.c_int),
+        "bld_set_field_physical": ([P, ctypes.c_int, D, ctypes.c_int], ctypes.c_int),
+        "bld_get_field_physical": ([P, ctypes.c_int, D, ctypes.c_int], ctypes.c_int),
         "bld_time_steps": ([P, ctypes.c_double, i64, ctypes.c_int, D], ctypes.c_int),
         "bld_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "bld_kernel_times": ([P, ctypes.c_int, D, ctypes.POINTER(i64)], ctypes.c_int),

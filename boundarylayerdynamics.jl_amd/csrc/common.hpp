@@ -101,6 +101,7 @@ struct Space {
     int wall_rough[2];
     double fstrain[2], fstress[2];
     double wbc[2];                    // vel3 Dirichlet values (physical)
+    int xfer_centered;                // transfer space: values at the cell centres (physical_space.jl:322-336)
     // learned closure (SGS spaces whose stresses come from the network)
     const unsigned short *mlp_w;      // bfloat16 W1, W2, W3
     const float *mlp_b;               // b1, b2, b3

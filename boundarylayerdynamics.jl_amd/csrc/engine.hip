@@ -144,8 +144,8 @@ struct EngineBase {
     virtual void project() = 0;
     virtual int init_problem() = 0;
     virtual int step(double dt, int64_t n, bool graph) = 0;
-    virtual void set_field_physical(int f, const double *v) = 0;
-    virtual void get_field_physical(int f, double *v) = 0;
+    virtual void set_field_physical(int f, const double *v, bool centered) = 0;
+    virtual void get_field_physical(int f, double *v, bool centered) = 0;
     virtual double time_steps(double dt, int64_t n, bool graph) = 0;
     virtual void set_profiling(bool on) = 0;
     virtual void kernel_time(int kind, double *ms, int64_t *count) = 0;
@@ -239,6 +239,8 @@ class Engine : public EngineBase {
         HIPCHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
         d_flag.alloc(sizeof(int));
         HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
         for (int i = 0; i < n; ++i) {
@@ -257,6 +259,8 @@ class Engine : public EngineBase {
         for (auto &ev : events) (void)hipEventDestroy(ev);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_fork2) (void)hipEventDestroy(ev_fork2);
+        if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -265,7 +269,7 @@ class Engine : public EngineBase {
     // side stream of the float64 (0,0)-mode stage pass, forked from and joined
     // back into `st` (also inside the captured graph)
     hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
     std::vector<std::unique_ptr<Instance<T>>> inst;
     int sel_inst = 0;
     TwiddleSet<T> tw;
@@ -866,18 +870,44 @@ class Engine : public EngineBase {
         for (const Pass &p : v) run_transform(kind, p, sel, kk);
     }
 
-    // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du
+    void fork(hipEvent_t ev) {
+        HIPCHECK(hipEventRecord(ev, st));
+        HIPCHECK(hipStreamWaitEvent(st2, ev, 0));
+    }
+    void join(hipEvent_t ev) {
+        HIPCHECK(hipEventRecord(ev, st2));
+        HIPCHECK(hipStreamWaitEvent(st, ev, 0));
+    }
+
+    // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du.
+    // Independent work overlaps on the side stream: the spectral linear terms
+    // with the c2r passes, the advection products with the SGS kernels; both
+    // join before the r2c passes (which read the products and accumulate into
+    // du).  With per-kernel profiling everything stays on the engine stream.
     void rhs(int sel) {
-        timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
+        const bool side = !profiling && !tp[PASS_C2R_X2].empty();
+        const Space<T> *dsp = d_spaces.as<Space<T>>();
+        if (side) {
+            fork(ev_fork);
+            launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st2);
+        } else {
+            timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
+        }
         if (tp[PASS_C2R_X2].empty()) return;
         run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
         run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
-        const Space<T> *dsp = d_spaces.as<Space<T>>();
         const int kks[5] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_SGS_MLP, KK_SGS_STRESS};
-        for (int k = 0; k < 3; ++k)
+        if (side && pp[PHYS_ADV].nblocks) {
+            fork(ev_fork2);
+            launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st2);
+        } else if (pp[PHYS_ADV].nblocks) {
+            timed(KK_ADV, [&] { launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st); });
+        }
+        for (int k = 1; k < 3; ++k)
             if (pp[k].nblocks) timed(kks[k], [&] { launch_physics<T>(k, dsp, pp[k].lt(), pp[k].nblocks, st); });
         for (int k = 3; k < 5; ++k)
             if (pp[k].nblocks) timed(kks[k], [&] { launch_closure<T>(k, dsp, pp[k].lt(), pp[k].nblocks, st); });
+        if (side) join(ev_join2);
         run_transform(PASS_R2C_X1, tp[PASS_R2C_X1], sel, KK_R2C_X1);
         run_transform(PASS_R2C_X2, tp[PASS_R2C_X2], sel, KK_R2C_X2);
         run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
@@ -1222,10 +1252,11 @@ class Engine : public EngineBase {
         finish(xfer[k]);
     }
 
-    void set_field_physical(int f, const double *v) override {
+    void set_field_physical(int f, const double *v, bool centered) override {
         if (f < 0 || f > 2) throw BldError{BLD_EARG, "field index must be 0, 1 or 2"};
         Instance<T> &I = cur();
         Space<T> &S = spaces[I.xfer];
+        S.xfer_centered = centered ? 1 : 0;
         const int nl = f < 2 ? I.n3 : I.n3 - 1;
         const size_t P = size_t(S.N1) * S.N2;
         std::vector<T> h(P * nl);
@@ -1348,10 +1379,11 @@ class Engine : public EngineBase {
         HIPCHECK(hipMemcpy(Y, dY.p, size_t(n) * mlp::PAD * sizeof(float), hipMemcpyDeviceToHost));
     }
 
-    void get_field_physical(int f, double *v) override {
+    void get_field_physical(int f, double *v, bool centered) override {
         if (f < 0 || f > 2) throw BldError{BLD_EARG, "field index must be 0, 1 or 2"};
         Instance<T> &I = cur();
         Space<T> &S = spaces[I.xfer];
+        S.xfer_centered = centered ? 1 : 0;
         const int nl = f < 2 ? I.n3 : I.n3 - 1;
         S.in_layers[0] = nl;
         S.in_job0[1] = nl;
@@ -1657,18 +1689,18 @@ int bld_step_graph(bld_engine *e, double dt, int64_t nsteps, double *t, double t
     })
 }
 
-int bld_set_field_physical(bld_engine *e, int field, const double *values) {
+int bld_set_field_physical(bld_engine *e, int field, const double *values, int centered) {
     ENGINE_CALL({
         if (!values) throw BldError{BLD_EARG, "null values"};
-        e->impl->set_field_physical(field, values);
+        e->impl->set_field_physical(field, values, centered != 0);
         return BLD_OK;
     })
 }
 
-int bld_get_field_physical(bld_engine *e, int field, double *values) {
+int bld_get_field_physical(bld_engine *e, int field, double *values, int centered) {
     ENGINE_CALL({
         if (!values) throw BldError{BLD_EARG, "null values"};
-        e->impl->get_field_physical(field, values);
+        e->impl->get_field_physical(field, values, centered != 0);
         return BLD_OK;
     })
 }

@@ -11,6 +11,16 @@ __device__ __forceinline__ cplx<T> czero() {
     return mk(T(0), T(0));
 }
 
+// exp(±iπ(k1/N1 + k2/N2)): moves physical samples by half a cell in x1 and x2
+// (shift!/unshift!, physical_space.jl:322-336)
+template <typename T>
+__device__ __forceinline__ cplx<T> centre_shift(const Space<T> &S, int k1, int i2, double sign) {
+    const int k2 = i2 <= (S.K2 - 1) / 2 ? i2 : i2 - S.K2;
+    double sn, cs;
+    sincospi(sign * (double(k1) / S.N1 + double(k2) / S.N2), &sn, &cs);
+    return mk(T(cs), T(sn));
+}
+
 // Spectral value of a c2r term at (k1, i2) of `layer` (derivatives.jl:404-418,566-568).
 template <typename T>
 __device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int recipe, int sel, int layer, int k1,
@@ -18,29 +28,34 @@ __device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int r
     const size_t P = size_t(S.K1) * S.K2;
     const size_t idx = size_t(layer) * P + k1 + size_t(S.K1) * i2;
     const cplx<T> *u = S.state[sel][0], *v = S.state[sel][1], *w = S.state[sel][2];
-    const T d1 = T(S.D1[k1]), d2 = T(S.D2[i2]);
+    // derivative factors are loaded only by the recipes that use them
+    auto d1 = [&] { return T(S.D1[k1]); };
+    auto d2 = [&] { return T(S.D2[i2]); };
     switch (recipe) {
         case R_U: return u[idx];
         case R_V: return v[idx];
         case R_W: return w[idx];
-        case R_D1U: return imul(d1, u[idx]);
-        case R_D2U: return imul(d2, u[idx]);
-        case R_D1V: return imul(d1, v[idx]);
-        case R_D2V: return imul(d2, v[idx]);
-        case R_D1W: return imul(d1, w[idx]);
-        case R_D2W: return imul(d2, w[idx]);
+        case R_D1U: return imul(d1(), u[idx]);
+        case R_D2U: return imul(d2(), u[idx]);
+        case R_D1V: return imul(d1(), v[idx]);
+        case R_D2V: return imul(d2(), v[idx]);
+        case R_D1W: return imul(d1(), w[idx]);
+        case R_D2W: return imul(d2(), w[idx]);
         case R_VORT1: {  // D2·w - (-α·v¯ + α·v⁺)
             const T a = T(S.alpha_i[layer + 1]);
             cplx<T> dz = (-a) * v[idx] + a * v[idx + P];
-            return imul(d2, w[idx]) - dz;
+            return imul(d2(), w[idx]) - dz;
         }
         case R_VORT2: {  // (-α·u¯ + α·u⁺) - D1·w
             const T a = T(S.alpha_i[layer + 1]);
             cplx<T> dz = (-a) * u[idx] + a * u[idx + P];
-            return dz - imul(d1, w[idx]);
+            return dz - imul(d1(), w[idx]);
         }
-        case R_VORT3: return imul(d1, v[idx]) - imul(d2, u[idx]);
-        default: return S.in_field[term][idx];
+        case R_VORT3: return imul(d1(), v[idx]) - imul(d2(), u[idx]);
+        default: {
+            const cplx<T> val = S.in_field[term][idx];
+            return S.xfer_centered ? cmul(val, centre_shift<T>(S, k1, i2, 1.0)) : val;
+        }
     }
 }
 

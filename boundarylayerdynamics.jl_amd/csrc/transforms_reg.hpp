@@ -312,6 +312,7 @@ __device__ __forceinline__ void r2c_x2_reg(const Space<T> &S, const LaunchEntry 
                     cplx<T> v = g[k2];
                     if (nH > 0) v = v + imul(T(S.D2[i2]), h[k2]);
                     const size_t o = size_t(i2) * S.K1 + k1_0 + c;
+                    if (S.xfer_centered) v = cmul(v, centre_shift<T>(S, k1_0 + c, i2, -1.0));
                     out[o] = out[o] + scale * v;
                 }
             });

@@ -14,6 +14,8 @@ are float64 Fortran-ordered (N1, N2, layers).
 
 import ctypes
 import math
+import os
+import warnings
 
 import numpy as np
 
@@ -531,13 +533,15 @@ class Model:
             return self.domain.hsize[1] * np.arange(n2) / n2
         return np.array([self.domain.vmap(z) for z in _zeta(self.n3, NODES[field])])
 
-    def set_physical(self, field, values):
+    def set_physical(self, field, values, centered=False):
+        """set_field! at the default physical size (physical_space.jl:253-269);
+        ``centered``: the values sit at the cell centres in x1 and x2."""
         a = np.asfortranarray(np.asarray(values, dtype=np.float64))
         n1, n2 = self.default_size()
         if a.shape != (n1, n2, self.nlayers(field)):
             raise ValueError("physical values must have the default physical size")
         self._use()
-        self._check(self._lib.bld_set_field_physical(self._h, FIELD_INDEX[field], dptr(a)))
+        self._check(self._lib.bld_set_field_physical(self._h, FIELD_INDEX[field], dptr(a), int(bool(centered))))
 
     def initialize(self, **ics):
         """State.jl:11-31: evaluate functions (or constants) on the default grid."""
@@ -556,11 +560,67 @@ class Model:
 
     def __getitem__(self, field):
         """Default-size physical values (State.jl:81-85)."""
+        return self.get_physical(field)
+
+    def get_physical(self, field, centered=False):
+        """get_field at the default physical size (physical_space.jl:297-312)."""
         n1, n2 = self.default_size()
         out = np.zeros((n1, n2, self.nlayers(field)), dtype=np.float64, order="F")
         self._use()
-        self._check(self._lib.bld_get_field_physical(self._h, FIELD_INDEX[field], dptr(out)))
+        self._check(self._lib.bld_get_field_physical(self._h, FIELD_INDEX[field], dptr(out), int(bool(centered))))
         return out
+
+    # -- snapshots (CBD files, fileio.jl; Snapshots, logging.jl:526-602) ------
+    def _xranges(self, field, centered):
+        from . import cbd
+        n1, n2 = self.default_size()
+        d = self.domain
+        x1 = cbd.hrange(d.hsize[0], n1, centered)
+        x2 = cbd.hrange(d.hsize[1], n2, centered)
+        x3 = np.array([d.vmap(z) for z in _zeta(self.n3, NODES[field])])
+        xmin = (0.0, 0.0, d.vmap(0.0))
+        xmax = (d.hsize[0], d.hsize[1], d.vmap(1.0))
+        return x1, x2, x3, xmin, xmax
+
+    def write_snapshot(self, path, centered=True, precision=None):
+        """One `state-NNN` folder of the Snapshots output (logging.jl:584-602):
+        a CBD file per field with the default-size physical values (cell
+        centres with ``centered``), in the model precision unless given."""
+        from . import cbd
+        dtype = np.dtype(precision if precision is not None else
+                         (np.float32 if self.precision == "fp32" else np.float64))
+        for f in self.fields:
+            x1, x2, x3, xmin, xmax = self._xranges(f, centered)
+            cbd.write_cbd(dtype, os.path.join(path, f"{f}.cbd"), self.get_physical(f, centered),
+                          x1, x2, x3, xmin, xmax)
+
+    def initialize_from(self, path):
+        """initialize!(model, path) (State.jl:33-66): read `<field>.cbd` of each
+        field (missing files set the field to zero), check that the file fits
+        the domain and grid, and set the state; the files must have the default
+        physical size of this model."""
+        from . import cbd
+        n1, n2 = self.default_size()
+        for f in self.fields:
+            fn = os.path.join(path, f"{f}.cbd")
+            if not os.path.isfile(fn):
+                warnings.warn(f"No initial values found for `{f}`, set to zero.")
+                self.set_physical(f, np.zeros((n1, n2, self.nlayers(f)), order="F"))
+                continue
+            xmin, xmax, x1, x2, x3, data = cbd.read_cbd(fn)
+            _, _, x3ref, xmin_ref, xmax_ref = self._xranges(f, False)
+            if not (np.allclose(xmin, xmin_ref) and np.allclose(xmax, xmax_ref)):
+                raise ValueError(f"{fn}: domain extent does not match the model")
+            if data.shape[:2] != (n1, n2):
+                raise NotImplementedError(f"{fn}: snapshot size {data.shape[:2]} differs from the default "
+                                          f"physical size {(n1, n2)}")
+            centered = x1[0] != 0
+            if not (np.allclose(x1, cbd.hrange(xmax[0] - xmin[0], n1, centered))
+                    and np.allclose(x2, cbd.hrange(xmax[1] - xmin[1], n2, centered))):
+                raise ValueError(f"{fn}: x1/x2 positions are not a uniform grid")
+            if x3.shape != x3ref.shape or not np.allclose(x3, x3ref):
+                raise ValueError(f"{fn}: x3 positions do not match the model grid")
+            self.set_physical(f, data, centered)
 
     # -- time integration ----------------------------------------------------
     def init_problem(self, t1=0.0):
@@ -656,22 +716,59 @@ def _approxdiv(a, b):  # util.jl:19-24
     return n
 
 
-def evolve(model, tspan, dt, method=None, graph=True):
-    """evolve! (BoundaryLayerDynamics.jl:190-219) without output modules.
+class Snapshots:
+    """Snapshots output (logging.jl:526-602): the state as CBD files in
+    `path`/state-NNN/ at every multiple NNN of `frequency` (the initial state is
+    not written), values at the cell centres unless ``centered=False``."""
+
+    def __init__(self, frequency, path=os.path.join("output", "snapshots"), centered=True, precision=None):
+        self.frequency, self.path, self.centered, self.precision = frequency, path, centered, precision
+
+    def due(self, t):
+        if t == 0:
+            return None
+        n = round(t / self.frequency)
+        return n if math.isclose(n, t / self.frequency, rel_tol=1e-9, abs_tol=1e-12) else None
+
+    def write(self, model, n):
+        model.write_snapshot(os.path.join(self.path, f"state-{n:03d}"), self.centered, self.precision)
+
+
+def evolve(model, tspan, dt, method=None, graph=True, output=()):
+    """evolve! (BoundaryLayerDynamics.jl:190-219).
 
     The state is projected and its rate evaluated at t1 (ODEProblem,
     ODEMethods.jl:179-185), then nt steps of `method` (default SSPRK33) run on
     the GPU, each followed by the rate evaluation of ODEMethods.jl:231.
-    Returns the final time."""
+    ``output``: Snapshots objects, written after the steps that reach their
+    times.  Returns the final time."""
     model.set_method(SSPRK33() if method is None else method)
     t1, t2 = (0.0, float(tspan)) if np.isscalar(tspan) else (float(tspan[0]), float(tspan[-1]))
     if not t1 <= t2:
         raise ValueError("Integration start time is larger than end time")
+    outputs = list(output) if isinstance(output, (list, tuple)) else [output]
+    for o in outputs:
+        if not isinstance(o, Snapshots):
+            raise NotImplementedError(f"output {type(o).__name__} is not supported by the GPU engine")
     model.init_problem(t1)
-    if t2 > t1:
-        nt = _approxdiv(t2 - t1, dt)
+    if t2 <= t1:
+        return t1
+    nt = _approxdiv(t2 - t1, dt)
+    if not outputs:
         return model.step(dt, nt, t=t1, t_end=t2, graph=graph)
-    return t1
+    # step to each output time in one engine call
+    t, done = t1, 0
+    while done < nt:
+        k = 1
+        while done + k < nt and not any(o.due(t1 + (done + k) * dt) for o in outputs):
+            k += 1
+        t = model.step(dt, k, t=t, t_end=t2, graph=graph)
+        done += k
+        for o in outputs:
+            n = o.due(t)
+            if n is not None:
+                o.write(model, n)
+    return t
 
 
 class Ensemble:

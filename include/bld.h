@@ -187,9 +187,11 @@ int bld_step_graph(bld_engine *e, double dt, int64_t nsteps, double *t, double t
 
 /* set_field!/get_field! at the default physical size (2+2k1max, 2+2k2max)
  * (src/physical_space.jl:253-312): `values` is real float64 [n1][n2][layers]
- * column-major (x1 fastest).  field: 0 vel1, 1 vel2, 2 vel3. */
-int bld_set_field_physical(bld_engine *e, int field, const double *values);
-int bld_get_field_physical(bld_engine *e, int field, double *values);
+ * column-major (x1 fastest).  field: 0 vel1, 1 vel2, 2 vel3.  centered != 0:
+ * the values sit at the cell centres (x_i + Δx/2), the `centered` keyword of
+ * the reference (shift!/unshift!, physical_space.jl:322-336). */
+int bld_set_field_physical(bld_engine *e, int field, const double *values, int centered);
+int bld_get_field_physical(bld_engine *e, int field, double *values, int centered);
 
 /* Engine timing hook: runs nsteps steps with inputs resident, returns the
  * elapsed milliseconds measured with HIP events on the engine stream. */

@@ -417,11 +417,18 @@ def test_ab2_requires_constant_dt():
 
 
 def test_constant_flux_keeps_the_mean():
-    """After every projection the z-weighted mean of the (0,0) mode of vel1 and
-    vel2 equals the prescribed flux (sources.jl:71-77)."""
-    ref, gpu = pair("channel_flux", seed=4)
+    """On a uniform grid (weights summing to one) every projection leaves the
+    z-weighted mean of the (0,0) mode of vel1 and vel2 at the prescribed flux
+    (sources.jl:71-77).  The reference adds the same shift to every layer, so on
+    a stretched grid, where the weights sum to one only approximately, a
+    residual remains; the stretched case is covered by the parity tests."""
+    mk = lambda M: M.Model((12, 10, 8), M.Domain((2 * np.pi, np.pi, 2.0), M.SmoothWall(), M.SmoothWall()),
+                           M.incompressible_flow(1 / 180, constant_flux=(1.0, 0.2)))
+    ref, gpu = mk(O), mk(B)
+    gpu.set_state(**random_state(ref, np.random.default_rng(4)))
     B.evolve(gpu, 0.01, 1e-3)
     st = gpu.state
     w = [pd for k, pd in ref.disc if k == "mean"][0]["weight"]
+    assert abs(np.sum(w) - 1.0) < 1e-14
     assert abs(np.sum(st["vel1"][0, 0, :].real * w) - 1.0) < 1e-12
     assert abs(np.sum(st["vel2"][0, 0, :].real * w) - 0.2) < 1e-12
