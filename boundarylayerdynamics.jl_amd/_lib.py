@@ -18,6 +18,9 @@ BLD_FP64, BLD_FP32_MIXED = 0, 1
 BLD_BC_DIRICHLET, BLD_BC_NEUMANN = 0, 1
 BLD_WALL_FREESLIP, BLD_WALL_ROUGH = 0, 1
 BLD_SSPRK33, BLD_SSPRK22, BLD_EULER, BLD_AB2 = 0, 1, 2, 3
+# mean-profile kinds (order of bld_profile)
+PROFILES = ("vel1", "vel2", "vel3", "adv1", "adv2", "adv3", "sgs11", "sgs12", "sgs13", "sgs22", "sgs23", "sgs33")
+I_PROFILES = ("vel3", "adv3", "sgs13", "sgs23")
 
 # kernel classes reported by bld_kernel_times (order of the engine's KernelKind)
 KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "sgs_stress", "r2c_x1",
@@ -31,7 +34,7 @@ EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_siz
            "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error",
            "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
            "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time",
-           "bld_sgs_closure", "bld_closure_sample", "bld_set_method")
+           "bld_sgs_closure", "bld_closure_sample", "bld_set_method", "bld_profiles_enable", "bld_profiles_read")
 
 
 class BldBC(ctypes.Structure):
@@ -113,6 +116,8 @@ def load():
         "bld_mlp_time": ([P, i64, ctypes.c_int, D, D], ctypes.c_int),
         "bld_sgs_closure": ([P, F, ctypes.c_double, D, ctypes.c_int], ctypes.c_int),
         "bld_set_method": ([P, ctypes.c_int], ctypes.c_int),
+        "bld_profiles_enable": ([P, ctypes.c_uint32], ctypes.c_int),
+        "bld_profiles_read": ([P, ctypes.c_int, D, ctypes.POINTER(i64), ctypes.c_int], ctypes.c_int),
         "bld_closure_sample": ([P, ctypes.c_int, ctypes.c_double, D, F, F, i64, ctypes.POINTER(i64)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():

@@ -6,7 +6,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SOURCES = ["csrc/kernels.hip", "csrc/engine.hip", "csrc/mlp.hip", "csrc/closure.hip"]
+SOURCES = ["csrc/kernels.hip", "csrc/engine.hip", "csrc/mlp.hip", "csrc/closure.hip", "csrc/profiles.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

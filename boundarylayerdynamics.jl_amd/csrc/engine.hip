@@ -153,6 +153,8 @@ struct EngineBase {
     virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
     virtual void set_closure(const float *params, double delta, const double *xw, int flags) = 0;
     virtual void set_method(int method) = 0;
+    virtual void profiles_enable(unsigned mask) = 0;
+    virtual int profiles_read(int kind, double *integral, int64_t *steps, bool reset) = 0;
     virtual void sample_closure(int F, double delta, const double *xw, float *X, float *Y, int64_t cap,
                                 int64_t *count) = 0;
 };
@@ -176,6 +178,9 @@ struct Instance {
     int ana = -1;                        // analysis space of the filtered-DNS sampler (created on demand)
     bool closure = false;                // SGS stresses from the learned closure
     DevBuf d_mlp_w, d_mlp_b, d_mlp_xw;
+    unsigned prof_mask = 0;              // enabled mean profiles (bit per ProfileKind)
+    int64_t prof_steps = 0;              // steps integrated since the last reset
+    DevBuf d_prof;                       // [2][PROF_COUNT][n3+1] last samples, integrals
     bool has_mean = false;               // ConstantMean projection on vel1/vel2
     CMeanDesc<T> cmean{};
     DevBuf d_mean_w;
@@ -943,6 +948,67 @@ class Engine : public EngineBase {
 
     // perform_step! of the selected method followed by rate! (ODEMethods.jl:100-166,231);
     // the Adams–Bashforth start step is taken outside graphs (ab2_start)
+    // mean-profile samples of every model that collects them, integrated with
+    // the trapezoidal weight w = dt/2 (0 for the first sample)
+    void sample_profiles(double w) {
+        for (auto &Ip : inst) {
+            Instance<T> &I = *Ip;
+            if (!I.prof_mask) continue;
+            ProfDesc<T> D{};
+            D.mask = I.prof_mask;
+            D.n3 = I.n3;
+            D.nk = I.nk;
+            for (int f = 0; f < 3; ++f) {
+                D.u[f] = I.d_u[f].template as<cplx<T>>();
+                D.u00[f] = I.mixed ? I.d_u00[f].template as<cplx<double>>() : nullptr;
+            }
+            D.adv = I.adv >= 0 ? d_spaces.as<Space<T>>() + I.adv : nullptr;
+            D.sgs = I.sgs >= 0 ? d_spaces.as<Space<T>>() + I.sgs : nullptr;
+            const size_t L = size_t(PROF_COUNT) * (I.n3 + 1);
+            D.prev = I.d_prof.template as<double>();
+            D.acc = D.prev + L;
+            launch_profile_sample<T>(D, w, st);
+        }
+    }
+
+    void profiles_enable(unsigned mask) override {
+        if (mask >> PROF_COUNT) throw BldError{BLD_EARG, "unknown profile"};
+        Instance<T> &I = cur();
+        const unsigned adv_bits = (1u << PROF_ADV1) | (1u << PROF_ADV2) | (1u << PROF_ADV3);
+        const unsigned sgs_bits = ((1u << PROF_COUNT) - 1) & ~((1u << PROF_ADV1) - 1) & ~adv_bits;
+        if ((mask & adv_bits) && I.adv < 0) throw BldError{BLD_EARG, "advection profiles need MomentumAdvection"};
+        if ((mask & sgs_bits) && I.sgs < 0) throw BldError{BLD_EARG, "SGS profiles need an SGS model"};
+        const size_t L = size_t(PROF_COUNT) * (I.n3 + 1);
+        if (mask && !I.d_prof.p) I.d_prof.alloc(2 * L * sizeof(double));
+        if (I.d_prof.p) HIPCHECK(hipMemset(I.d_prof.p, 0, 2 * L * sizeof(double)));
+        I.prof_mask = mask;
+        I.prof_steps = 0;
+        drop_graph();
+        if (mask) {  // first sample of the current state (the rate is evaluated first)
+            rhs(0);
+            sample_profiles(0.0);
+            HIPCHECK(hipStreamSynchronize(st));
+        }
+    }
+
+    int profiles_read(int kind, double *integral, int64_t *steps, bool reset) override {
+        Instance<T> &I = cur();
+        if (kind < 0 || kind >= PROF_COUNT) throw BldError{BLD_EARG, "unknown profile"};
+        if (!(I.prof_mask & (1u << kind))) throw BldError{BLD_EARG, "profile not enabled"};
+        const bool inode = kind == PROF_VEL3 || kind == PROF_ADV3 || kind == PROF_SGS13 || kind == PROF_SGS23;
+        const int len = inode ? I.n3 + 1 : I.n3;
+        const size_t L = size_t(PROF_COUNT) * (I.n3 + 1);
+        double *acc = I.d_prof.template as<double>() + L + size_t(kind) * (I.n3 + 1);
+        HIPCHECK(hipStreamSynchronize(st));
+        if (integral) HIPCHECK(hipMemcpy(integral, acc, len * sizeof(double), hipMemcpyDeviceToHost));
+        if (steps) *steps = I.prof_steps;
+        if (reset) {
+            HIPCHECK(hipMemset(I.d_prof.template as<double>() + L, 0, L * sizeof(double)));
+            I.prof_steps = 0;
+        }
+        return len;
+    }
+
     void one_step(double dt) {
         switch (method) {
             case BLD_EULER:
@@ -967,6 +1033,7 @@ class Engine : public EngineBase {
                 stage(STAGE_RK3_3, dt);
                 rhs(0);
         }
+        sample_profiles(0.5 * dt);
     }
 
     // the first Adams–Bashforth step is a two-stage Runge-Kutta step that also
@@ -981,6 +1048,8 @@ class Engine : public EngineBase {
         rhs(0);
         stage(STAGE_AB2_FIRST_2, dt);
         rhs(0);
+        sample_profiles(0.5 * dt);
+        for (auto &I : inst) I->prof_steps += I->prof_mask ? 1 : 0;
         ab2_started = true;
         ab2_dt = dt;
         return n - 1;
@@ -1069,6 +1138,7 @@ class Engine : public EngineBase {
         ab2_started = false;
         stage(0, 0.0);
         rhs(0);
+        sample_profiles(0.0);
         check_du();
         return read_flag() ? BLD_EDIVERGED : BLD_OK;
     }
@@ -1101,6 +1171,7 @@ class Engine : public EngineBase {
             else
                 one_step(dt);
         }
+        for (auto &I : inst) I->prof_steps += I->prof_mask ? n : 0;
         check_du();
         if (profiling) collect_profile();
         return read_flag() ? BLD_EDIVERGED : BLD_OK;
@@ -1312,6 +1383,7 @@ class Engine : public EngineBase {
         Space<T> &S = spaces[I.sgs];
         if (!params) {
             I.closure = false;
+            S.mlp_w = nullptr;
         } else {
             if (!xw || !(delta > 0)) throw BldError{BLD_EARG, "learned closure: missing wall feature or Δ <= 0"};
             for (int i = 0; i < mlp::NPARAM; ++i)
@@ -1593,6 +1665,20 @@ int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wa
     if (!count) return BLD_EARG;
     ENGINE_CALL({
         e->impl->sample_closure(filter, delta, wall_feature, x, t, capacity, count);
+        return BLD_OK;
+    })
+}
+
+int bld_profiles_enable(bld_engine *e, uint32_t mask) {
+    ENGINE_CALL({
+        e->impl->profiles_enable(mask);
+        return BLD_OK;
+    })
+}
+
+int bld_profiles_read(bld_engine *e, int profile, double *integral, int64_t *steps, int reset) {
+    ENGINE_CALL({
+        e->impl->profiles_read(profile, integral, steps, reset != 0);
         return BLD_OK;
     })
 }

@@ -69,6 +69,23 @@ __host__ __device__ constexpr bool stage_reads_ut(int s) {
 }
 __host__ __device__ constexpr bool stage_saves_du(int s) { return s == STAGE_AB2 || s == STAGE_AB2_FIRST_1; }
 
+// Mean profiles of one model (MeanProfiles, logging.jl:310-525)
+enum ProfileKind : int {
+    PROF_VEL1 = 0, PROF_VEL2, PROF_VEL3, PROF_ADV1, PROF_ADV2, PROF_ADV3,
+    PROF_SGS11, PROF_SGS12, PROF_SGS13, PROF_SGS22, PROF_SGS23, PROF_SGS33, PROF_COUNT
+};
+template <typename T>
+struct ProfDesc {
+    unsigned mask;
+    int n3, nk;
+    const cplx<T> *u[3];
+    const cplx<double> *u00[3];      // float64 (0,0) shadows of the mixed form (else null)
+    const Space<T> *adv, *sgs;       // device pointers of the model's spaces (or null)
+    double *prev, *acc;              // [PROF_COUNT][n3+1]: last sample, time integral
+};
+template <typename T>
+void launch_profile_sample(const ProfDesc<T> &D, double weight, hipStream_t st);
+
 // ConstantMean projection of one model (vel1, vel2 at C-nodes)
 template <typename T>
 struct CMeanDesc {

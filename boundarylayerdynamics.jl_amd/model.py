@@ -570,6 +570,32 @@ class Model:
         self._check(self._lib.bld_get_field_physical(self._h, FIELD_INDEX[field], dptr(out), int(bool(centered))))
         return out
 
+    # -- mean profiles on the device (MeanProfiles, logging.jl:310-525) ---------
+    def enable_profiles(self, names):
+        """Collect the named profiles (see MeanProfiles) from now on; () stops."""
+        mask = 0
+        for n in names:
+            if n not in _lib.PROFILES:
+                raise ValueError(f"unknown profile {n!r}")
+            mask |= 1 << _lib.PROFILES.index(n)
+        self._use()
+        self._check(self._lib.bld_profiles_enable(self._h, mask))
+        self._profiles = tuple(names)
+
+    def read_profiles(self, reset=False):
+        """({name: time integral since the last reset}, steps integrated)."""
+        out, steps = {}, ctypes.c_int64()
+        self._use()
+        for n in getattr(self, "_profiles", ()):
+            k = _lib.PROFILES.index(n)
+            a = np.zeros(self.n3 + 1 if n in _lib.I_PROFILES else self.n3)
+            self._check(self._lib.bld_profiles_read(self._h, k, dptr(a), ctypes.byref(steps), 0))
+            out[n] = a
+        if reset and out:
+            self._check(self._lib.bld_profiles_read(self._h, _lib.PROFILES.index(self._profiles[0]), None,
+                                                    ctypes.byref(steps), 1))
+        return out, steps.value
+
     # -- snapshots (CBD files, fileio.jl; Snapshots, logging.jl:526-602) ------
     def _xranges(self, field, centered):
         from . import cbd
@@ -716,6 +742,46 @@ def _approxdiv(a, b):  # util.jl:19-24
     return n
 
 
+class MeanProfiles:
+    """MeanProfiles output (logging.jl:310-525): time averages (trapezoidal
+    rule over the samples after each step) of horizontal means, collected on
+    the GPU.  ``profiles`` from vel1-3, adv1-3, sgs11, sgs12, sgs13, sgs22,
+    sgs23, sgs33; at every multiple of ``output_frequency`` (and at the end)
+    the averages since the previous output are written to `path`-NN.npz, one
+    array per profile plus `intervals` and `timespan` (the reference writes
+    the same datasets and attributes to HDF5, which is not available here)."""
+
+    def __init__(self, profiles=("vel1", "vel2", "vel3"), path=os.path.join("output", "profiles"),
+                 output_frequency=None):
+        self.profiles, self.path, self.output_frequency = tuple(profiles), path, output_frequency
+
+    def due(self, t):
+        if self.output_frequency is None or t == 0:
+            return False
+        n = round(t / self.output_frequency)
+        return math.isclose(n, t / self.output_frequency, rel_tol=1e-9, abs_tol=1e-12)
+
+    def start(self, model, t):
+        model.enable_profiles(self.profiles)
+        self.t0 = t
+
+    def flush(self, model, t):
+        integrals, steps = model.read_profiles(reset=True)
+        if steps == 0 or not t > self.t0:
+            return None
+        means = {k: v / (t - self.t0) for k, v in integrals.items() if k in self.profiles}
+        d = os.path.dirname(self.path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        i = 1
+        while os.path.exists(f"{self.path}-{i:02d}.npz"):
+            i += 1
+        fn = f"{self.path}-{i:02d}.npz"
+        np.savez(fn, intervals=np.int64(steps), timespan=np.array([self.t0, t]), **means)
+        self.t0 = t
+        return fn
+
+
 class Snapshots:
     """Snapshots output (logging.jl:526-602): the state as CBD files in
     `path`/state-NNN/ at every multiple NNN of `frequency` (the initial state is
@@ -748,9 +814,14 @@ def evolve(model, tspan, dt, method=None, graph=True, output=()):
         raise ValueError("Integration start time is larger than end time")
     outputs = list(output) if isinstance(output, (list, tuple)) else [output]
     for o in outputs:
-        if not isinstance(o, Snapshots):
+        if not isinstance(o, (Snapshots, MeanProfiles)):
             raise NotImplementedError(f"output {type(o).__name__} is not supported by the GPU engine")
+    if sum(isinstance(o, MeanProfiles) for o in outputs) > 1:
+        raise NotImplementedError("one MeanProfiles output per model")
     model.init_problem(t1)
+    for o in outputs:
+        if isinstance(o, MeanProfiles):
+            o.start(model, t1)
     if t2 <= t1:
         return t1
     nt = _approxdiv(t2 - t1, dt)
@@ -765,9 +836,16 @@ def evolve(model, tspan, dt, method=None, graph=True, output=()):
         t = model.step(dt, k, t=t, t_end=t2, graph=graph)
         done += k
         for o in outputs:
-            n = o.due(t)
-            if n is not None:
-                o.write(model, n)
+            if isinstance(o, Snapshots):
+                n = o.due(t)
+                if n is not None:
+                    o.write(model, n)
+            elif o.due(t):
+                o.flush(model, t)
+    for o in outputs:  # flush! at the end of the run (logging.jl:453-491)
+        if isinstance(o, MeanProfiles):
+            o.flush(model, t)
+            model.enable_profiles(())
     return t
 
 

@@ -155,6 +155,21 @@ int bld_closure_sample(bld_engine *e, int filter, double delta, const double *wa
 typedef enum bld_method { BLD_SSPRK33 = 0, BLD_SSPRK22 = 1, BLD_EULER = 2, BLD_AB2 = 3 } bld_method;
 int bld_set_method(bld_engine *e, int method);
 
+/* Mean profiles (MeanProfiles, logging.jl:310-525) of the selected model,
+ * accumulated on the device: horizontal means sampled at the rate evaluation
+ * that ends each step (and when enabled or the problem is initialised), time
+ * integrated with the trapezoidal rule.  mask: one bit per bld_profile, 0
+ * disables.  bld_profiles_read copies the time integral since the last reset
+ * (n3 values for C-node profiles; n3+1 for I-node profiles, boundary entries
+ * included) and the number of steps it covers; the mean is the integral
+ * divided by the time span. */
+typedef enum bld_profile {
+    BLD_PROF_VEL1 = 0, BLD_PROF_VEL2, BLD_PROF_VEL3, BLD_PROF_ADV1, BLD_PROF_ADV2, BLD_PROF_ADV3,
+    BLD_PROF_SGS11, BLD_PROF_SGS12, BLD_PROF_SGS13, BLD_PROF_SGS22, BLD_PROF_SGS23, BLD_PROF_SGS33
+} bld_profile;
+int bld_profiles_enable(bld_engine *e, uint32_t mask);
+int bld_profiles_read(bld_engine *e, int profile, double *integral, int64_t *steps, int reset);
+
 /* Message of the last failed call on `e` (or of the last failed bld_create
  * on this thread when e is NULL). */
 const char *bld_last_error(const bld_engine *e);

@@ -730,6 +730,7 @@ def _sgs(model, rates, state, pdata):
     fst_l, fst_u = pdata["fstress"]
     mag_l = np.sqrt(u[:, :, 0] ** 2 + v[:, :, 0] ** 2)
     mag_u = np.sqrt(u[:, :, -1] ** 2 + v[:, :, -1] ** 2)
+    out = dict(evc=evc, evi=evi, sgs11=t11, sgs22=t22, sgs33=t33, sgs12=t12)
     for t_dir, Sx, vel, rate_f, rate_h, pre in ((1, S13, u, "vel1", "vel3", D1), (2, S23, v, "vel2", "vel3", D2)):
         ti = -2 * evi * Sx
         add_field(rates[rate_h], -ti, pre)
@@ -739,11 +740,14 @@ def _sgs(model, rates, state, pdata):
         ext = np.concatenate([bl[:, :, None], ti, bu[:, :, None]], axis=2)
         div = -ac * ext[:, :, :-1] + ac * ext[:, :, 1:]
         add_field(rates[rate_f], -div)
-    return dict(evc=evc, evi=evi)
+        out[f"sgs{t_dir}3"] = ext  # I-node stress with the wall stresses as boundary layers
+    return out
 
 
-def compute_rates(model, rates, state):
-    """Processes.jl:39-62: reset, linear (spectral) terms, nonlinear (physical) terms."""
+def compute_rates(model, rates, state, terms=None):
+    """Processes.jl:39-62: reset, linear (spectral) terms, nonlinear (physical) terms.
+    ``terms`` (a dict) receives the physical-space terms that the reference logs
+    (log_sample!: adv1-3, sgs11-33)."""
     for r in rates.values():
         r[...] = 0
     for kind, pd in model.disc:  # linear processes first
@@ -753,9 +757,13 @@ def compute_rates(model, rates, state):
             rates[pd["field"]][0, 0, :] += pd["strength"]  # sources.jl:32-35
     for kind, pd in model.disc:
         if kind == "adv":
-            _advection(model, rates, state, pd)
+            r = _advection(model, rates, state, pd)
         elif kind == "sgs":
-            _sgs(model, rates, state, pd)
+            r = _sgs(model, rates, state, pd)
+        else:
+            continue
+        if terms is not None:
+            terms.update(r)
     return rates
 
 

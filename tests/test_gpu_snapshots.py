@@ -90,3 +90,70 @@ def test_snapshots_output_in_evolve(tmp_path):
     assert sorted(p.name for p in (tmp_path / "snaps").iterdir()) == ["state-001", "state-002", "state-003"]
     _, _, _, _, _, data = B.cbd.read_cbd(str(tmp_path / "snaps" / "state-003" / "vel1.cbd"))
     assert np.allclose(data, m["vel1"], rtol=0, atol=1e-14)
+
+
+def _les_pair(M, n=(16, 16, 8)):
+    dom = M.Domain((2 * math.pi, 2 * math.pi, 1.0), M.RoughWall(1e-4, 0.4), M.FreeSlipBoundary())
+    return M.Model(n, dom, M.incompressible_flow(1e-5, constant_forcing=1,
+                                                 sgs_model=M.StaticSmagorinskyModel(Cs=0.1, wall_damping=True)))
+
+
+@pytest.mark.parametrize("precision,rtol", [("fp64", 1e-10), ("fp32", 2e-4)])
+def test_mean_profiles_match_oracle(precision, rtol):
+    """All 12 profiles integrated on the device over 4 steps against the
+    oracle's samples after each step, trapezoidal rule."""
+    import oracle as O
+    from oracle import profiles as OP
+    ref = _les_pair(O)
+    gpu = _les_pair(B) if precision == "fp64" else B.Model((16, 16, 8), _les_pair(B).domain,
+                                                            _les_pair(B).processes, precision="fp32")
+    rng = np.random.default_rng(5)
+    W = B.workloads
+    n1, n2 = gpu.default_size()
+    u, v = W.les_initial(gpu.coordinates("vel1", 3), n1, n2, seed=5)
+    gpu.set_physical("vel1", u)
+    gpu.set_physical("vel2", v)
+    gpu.set_physical("vel3", np.zeros((n1, n2, 7), order="F"))
+    gpu.project()
+    st = gpu.state
+    for f in ref.fields:
+        ref.state[f][...] = st[f]
+    dt, steps = 5e-4, 4
+    names = OP.NAMES
+    gpu.init_problem()
+    gpu.enable_profiles(names)
+    gpu.step(dt, steps)
+    got, nsteps = gpu.read_profiles()
+    assert nsteps == steps
+    samples = [OP.samples(ref, ref.state)]
+    for k in range(steps):
+        O.evolve(ref, (k * dt, (k + 1) * dt), dt)
+        samples.append(OP.samples(ref, ref.state))
+    want = OP.trapezoid(samples, dt)
+    for n in names:
+        scale = max(np.abs(want[n]).max(), 1e-300)
+        assert np.abs(got[n] - want[n]).max() / scale < rtol, (n, np.abs(got[n] - want[n]).max() / scale)
+
+
+def test_mean_profiles_output_files(tmp_path):
+    m = _les_pair(B)
+    n1, n2 = m.default_size()
+    u, v = B.workloads.les_initial(m.coordinates("vel1", 3), n1, n2, seed=6)
+    m.set_physical("vel1", u)
+    m.set_physical("vel2", v)
+    out = B.MeanProfiles(("vel1", "sgs13"), path=str(tmp_path / "prof" / "profiles"), output_frequency=2e-3)
+    B.evolve(m, 4e-3, 5e-4, output=out)
+    files = sorted(p.name for p in (tmp_path / "prof").iterdir())
+    assert files == ["profiles-01.npz", "profiles-02.npz"]
+    d = np.load(str(tmp_path / "prof" / "profiles-02.npz"))
+    assert int(d["intervals"]) == 4 and np.allclose(d["timespan"], [2e-3, 4e-3])
+    assert d["vel1"].shape == (8,) and d["sgs13"].shape == (9,)
+    assert np.all(d["vel1"] > 0) and d["sgs13"][0] < 0 and d["sgs13"][-1] == 0
+
+
+def test_profiles_need_their_processes():
+    dom = B.Domain((1.0, 1.0, 1.0), B.SmoothWall(), B.SmoothWall())
+    m = B.Model((8, 8, 6), dom, B.incompressible_flow(0.1))
+    m.enable_profiles(("vel1", "adv3"))
+    with pytest.raises(B._lib.BldError, match="SGS"):
+        m.enable_profiles(("sgs11",))
