@@ -62,23 +62,20 @@ __device__ __forceinline__ void velocity_gradient(const Space<T> &S, size_t P, i
 
 // ---------------------------------------------------------------------------
 // learned stresses at the C-nodes: one thread per point for the features and
-// the stress, one wave per 64 points on MFMA for the network
+// the stress, one wave per 64 points on MFMA for the network (weights and
+// activations in registers, forward_regs)
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_sgs_mlp(const Space<T> *__restrict__ spaces, LaunchTable lt) {
-    __shared__ unsigned short sW[NWB];
-    __shared__ float sB[2 * NH + PAD];
-    __shared__ unsigned short sX[256 * PAD];
-    __shared__ float sY[256 * 8];
-    __shared__ unsigned short sT[4][16 * NH];
+    __shared__ __align__(16) unsigned short sX[256 * PAD];  // vector reads of 4 bf16
+    __shared__ __align__(16) float sY[256 * 8];             // vector writes of 4 floats
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-    for (int i = threadIdx.x; i < NWB; i += 256) sW[i] = S.mlp_w[i];
-    for (int i = threadIdx.x; i < 2 * NH + PAD; i += 256) sB[i] = S.mlp_b[i];
-    __syncthreads();
     const size_t P = size_t(S.N1) * S.N2, npts = P * S.n3;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, gq = lane >> 4;
+    NetRegs net;  // weights and biases stay in registers for all tiles of the wave
+    load_net(S.mlp_w, S.mlp_b, lane, net);
     for (int c = 0; c < kMlpChunks; ++c) {
         const size_t gid = (size_t(blockIdx.x - E.block_begin) * kMlpChunks + c) * 256 + threadIdx.x;
         const bool live = gid < npts;
@@ -103,13 +100,10 @@ __global__ void __launch_bounds__(256) k_sgs_mlp(const Space<T> *__restrict__ sp
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const bf16x4 xa = a_frag(sX + (64 * wave + 16 * s) * PAD, PAD, 0, lane);
-            Fwd o;
-            forward_tile(xa, sW, sB, sT[wave], lane, o);
-            if (r < 8) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) sY[(64 * wave + 16 * s + 4 * gq + j) * 8 + r] = o.y[j];
-            }
+            // lane holds x[point r][feature 4g + j]: one 8-byte read per lane
+            const bf16x4 xb = *reinterpret_cast<const bf16x4 *>(sX + (64 * wave + 16 * s + r) * PAD + 4 * gq);
+            const floatx4 y = forward_regs(net, xb);  // lane: y[point r][output 4g + j]
+            if (gq < 2) *reinterpret_cast<floatx4 *>(sY + (64 * wave + 16 * s + r) * 8 + 4 * gq) = y;
         }
         __syncthreads();
         if (live) {

@@ -36,6 +36,33 @@ __device__ __forceinline__ cplx<T> cmulc(cplx<T> a, cplx<T> b) {  // a * conj(b)
 template <typename T>
 __device__ __forceinline__ cplx<T> imul(T d, cplx<T> a) { return mk(-d * a.y, d * a.x); }
 
+// Loads and stores through the global address space.  Pointers read from the
+// device descriptors are generic to the compiler, whose flat instructions also
+// wait on the LDS counter; global instructions let the loads of a thread stay
+// in flight across LDS traffic.
+template <typename T>
+__device__ __forceinline__ T gload(const T *p) {
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> gload(const cplx<T> *p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = *(const __attribute__((address_space(1))) v2 *)p;
+    return mk(v.x, v.y);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, T v) {
+    *(__attribute__((address_space(1))) T *)p = v;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(cplx<T> *p, cplx<T> v) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w;
+    w.x = v.x;
+    w.y = v.y;
+    *(__attribute__((address_space(1))) v2 *)p = w;
+}
+
 constexpr int kMaxStages = 12;
 
 // Mixed-radix Stockham plan for one transform length.

@@ -36,22 +36,16 @@ __device__ __forceinline__ bf16x4 load_x(const float *__restrict__ X, int64_t p0
 // inference: Y[n][16] = net(X[n][16])
 __global__ void __launch_bounds__(256) k_forward(const unsigned short *__restrict__ wb, const float *__restrict__ p,
                                                  const float *__restrict__ X, float *__restrict__ Y, int64_t n) {
-    __shared__ unsigned short sW[NWB];
-    __shared__ float sB[2 * NH + PAD];
-    __shared__ unsigned short sT[WAVES][16 * NH];
-    load_params(wb, p, sW, sB);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
+    NetRegs net;
+    load_net(wb, p + OB1, lane, net);
     const int64_t ntiles = (n + 15) / 16;
     for (int64_t t = int64_t(blockIdx.x) * WAVES + wave; t < ntiles; t += int64_t(gridDim.x) * WAVES) {
         const bf16x4 xa = load_x(X, t * 16, n, lane);
-        Fwd o;
-        forward_tile(xa, sW, sB, sT[wave], lane, o);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t pt = t * 16 + 4 * g + j;
-            if (pt < n) Y[pt * PAD + r] = o.y[j];
-        }
+        const floatx4 y = forward_regs(net, xa);
+        const int64_t pt = t * 16 + r;
+        if (pt < n) *reinterpret_cast<floatx4 *>(Y + pt * PAD + 4 * g) = y;
     }
 }
 

@@ -145,5 +145,65 @@ __device__ __forceinline__ void forward_tile(const bf16x4 &xa, const unsigned sh
     (void)g;
 }
 
+// Register-resident forward pass (inference).  Each layer is computed
+// transposed, Zᵀ = Wᵀ·Xᵀ, with the 16 points of a tile along N: the A operand
+// is a 16x16 block of Wᵀ (held in registers for the whole kernel), the B
+// operand the activations, and the D layout of a layer's output (lane holds
+// Z[point r][neuron 16t + 4g + j]) is exactly the B layout the next layer
+// needs, so activations never leave the registers.
+struct NetRegs {
+    bf16x4 w1[4], w2[4][4], w3[4];  // Wᵀ blocks: [out block][in block]
+    float b1[4][4], b2[4][4], b3[4];  // biases of the D rows 16t + 4g + j
+};
+
+// W: bfloat16 weights (W1, W2, W3 row-major [in][out]); bias: b1, b2, b3 (float)
+__device__ __forceinline__ void load_net(const unsigned short *__restrict__ W, const float *__restrict__ bias,
+                                         int lane, NetRegs &n) {
+    const int r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            n.w1[t][j] = short(W[OW1 + (4 * g + j) * NH + 16 * t + r]);
+            n.w3[t][j] = short(W[OW3 + (16 * t + 4 * g + j) * PAD + r]);
+            n.b1[t][j] = bias[16 * t + 4 * g + j];
+            n.b2[t][j] = bias[NH + 16 * t + 4 * g + j];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) n.w2[t][c][j] = short(W[OW2 + (16 * c + 4 * g + j) * NH + 16 * t + r]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) n.b3[j] = bias[2 * NH + 4 * g + j];
+}
+
+// xb: lane holds x[point r][feature 4g + j] (bf16); returns y with lane holding
+// y[point r][output 4g + j]
+__device__ __forceinline__ floatx4 forward_regs(const NetRegs &n, const bf16x4 &xb) {
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+    bf16x4 h1[4], h2[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        floatx4 acc = mfma(n.w1[t], xb, zero);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h1[t][j] = short(to_bf16(gelu(acc[j] + n.b1[t][j])));
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        floatx4 acc = zero;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc = mfma(n.w2[t][c], h1[c], acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) h2[t][j] = short(to_bf16(gelu(acc[j] + n.b2[t][j])));
+    }
+    floatx4 y = zero;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) y = mfma(n.w3[c], h2[c], y);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) y[j] += n.b3[j];
+    return y;
+}
+
 }  // namespace mlp
 }  // namespace bld

@@ -29,31 +29,31 @@ __device__ __forceinline__ cplx<T> term_value(const Space<T> &S, int term, int r
     const size_t idx = size_t(layer) * P + k1 + size_t(S.K1) * i2;
     const cplx<T> *u = S.state[sel][0], *v = S.state[sel][1], *w = S.state[sel][2];
     // derivative factors are loaded only by the recipes that use them
-    auto d1 = [&] { return T(S.D1[k1]); };
-    auto d2 = [&] { return T(S.D2[i2]); };
+    auto d1 = [&] { return T(gload(S.D1 + k1)); };
+    auto d2 = [&] { return T(gload(S.D2 + i2)); };
     switch (recipe) {
-        case R_U: return u[idx];
-        case R_V: return v[idx];
-        case R_W: return w[idx];
-        case R_D1U: return imul(d1(), u[idx]);
-        case R_D2U: return imul(d2(), u[idx]);
-        case R_D1V: return imul(d1(), v[idx]);
-        case R_D2V: return imul(d2(), v[idx]);
-        case R_D1W: return imul(d1(), w[idx]);
-        case R_D2W: return imul(d2(), w[idx]);
+        case R_U: return gload(u + idx);
+        case R_V: return gload(v + idx);
+        case R_W: return gload(w + idx);
+        case R_D1U: return imul(d1(), gload(u + idx));
+        case R_D2U: return imul(d2(), gload(u + idx));
+        case R_D1V: return imul(d1(), gload(v + idx));
+        case R_D2V: return imul(d2(), gload(v + idx));
+        case R_D1W: return imul(d1(), gload(w + idx));
+        case R_D2W: return imul(d2(), gload(w + idx));
         case R_VORT1: {  // D2·w - (-α·v¯ + α·v⁺)
-            const T a = T(S.alpha_i[layer + 1]);
-            cplx<T> dz = (-a) * v[idx] + a * v[idx + P];
-            return imul(d2(), w[idx]) - dz;
+            const T a = T(gload(S.alpha_i + layer + 1));
+            cplx<T> dz = (-a) * gload(v + idx) + a * gload(v + idx + P);
+            return imul(d2(), gload(w + idx)) - dz;
         }
         case R_VORT2: {  // (-α·u¯ + α·u⁺) - D1·w
-            const T a = T(S.alpha_i[layer + 1]);
-            cplx<T> dz = (-a) * u[idx] + a * u[idx + P];
-            return dz - imul(d1(), w[idx]);
+            const T a = T(gload(S.alpha_i + layer + 1));
+            cplx<T> dz = (-a) * gload(u + idx) + a * gload(u + idx + P);
+            return dz - imul(d1(), gload(w + idx));
         }
-        case R_VORT3: return imul(d1(), v[idx]) - imul(d2(), u[idx]);
+        case R_VORT3: return imul(d1(), gload(v + idx)) - imul(d2(), gload(u + idx));
         default: {
-            const cplx<T> val = S.in_field[term][idx];
+            const cplx<T> val = gload(S.in_field[term] + idx);
             return S.xfer_centered ? cmul(val, centre_shift<T>(S, k1, i2, 1.0)) : val;
         }
     }

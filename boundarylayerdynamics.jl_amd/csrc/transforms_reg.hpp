@@ -20,6 +20,14 @@
 
 namespace bld {
 
+// twiddle exp(∓2πi t/N) from the table of the length (conjugated for INV)
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> gtwiddle(const cplx<T> *__restrict__ tw, int t) {
+    cplx<T> w = gload(tw + t);
+    if (INV) w.y = -w.y;
+    return w;
+}
+
 // LDS entries (complex) per batch item of a transform pass with a register
 // plan: x2 passes one column per region (r2c_x2: G and H sums), x1 passes the
 // padded rows of a pair (r2c_x1: and the pair's spectrum)
@@ -45,27 +53,48 @@ __device__ __forceinline__ void c2r_x2_reg(const Space<T> &S, const LaunchEntry 
     const int k1_0 = chunk * C;
     const int ncols = min(C, S.nk1 - k1_0);
     const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;
+    const size_t P = size_t(S.K1) * S.K2;
     for (int task = threadIdx.x; task < C * Bq; task += blockDim.x) {
         const int c = task & (C - 1), t = task >> lc;
+        const int k1 = k1_0 + min(c, ncols - 1);  // clamped: every load below is in bounds
         cplx<T> v[A];
+        // indices of the A values (resolved modes only; the padding is zero):
+        // all loads are issued before any is used, no branches around them
+        int i2s[A];
+        bool ok[A];
         static_for<0, A>([&](auto n1c) {
             constexpr int n1 = decltype(n1c)::value;
             const int pos = Bq * n1 + t;
-            cplx<T> val = czero<T>();
-            if (c < ncols) {
-                int i2 = -1;
-                if (pos <= S.k2m)
-                    i2 = pos;
-                else if (pos >= N - S.k2m)
-                    i2 = S.K2 - (N - pos);
-                if (i2 >= 0) val = term_value(S, term, recipe, sel, layer, k1_0 + c, i2);
-            }
-            v[n1] = val;
+            const bool lo = pos <= S.k2m, hi = pos >= N - S.k2m;
+            ok[n1] = (lo || hi) && c < ncols;
+            i2s[n1] = lo ? pos : (hi ? S.K2 - (N - pos) : 0);
         });
+        const size_t base = size_t(layer) * P + k1;
+        auto gather = [&](auto value) {
+            static_for<0, A>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                const cplx<T> x = value(base + size_t(S.K1) * i2s[n1], i2s[n1]);
+                v[n1] = ok[n1] ? x : czero<T>();
+            });
+        };
+        const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
+        const T d1 = T(gload(S.D1 + k1));
+        switch (recipe) {  // uniform per block
+            case R_U: gather([&](size_t i, int) { return gload(u + i); }); break;
+            case R_V: gather([&](size_t i, int) { return gload(vv + i); }); break;
+            case R_W: gather([&](size_t i, int) { return gload(w + i); }); break;
+            case R_D1U: gather([&](size_t i, int) { return imul(d1, gload(u + i)); }); break;
+            case R_D1V: gather([&](size_t i, int) { return imul(d1, gload(vv + i)); }); break;
+            case R_D1W: gather([&](size_t i, int) { return imul(d1, gload(w + i)); }); break;
+            case R_D2U: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(u + i)); }); break;
+            case R_D2V: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(vv + i)); }); break;
+            case R_D2W: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(w + i)); }); break;
+            default: gather([&](size_t i, int i2) { return term_value(S, term, recipe, sel, layer, k1, i2); });
+        }
         rdft<T, true, A>(v);
         static_for<0, A>([&](auto k1c) {
             constexpr int k1 = decltype(k1c)::value;
-            const cplx<T> y = k1 == 0 ? v[0] : cmul(v[k1], twiddle<T, true>(tw, t * k1));
+            const cplx<T> y = k1 == 0 ? v[0] : cmul(v[k1], gtwiddle<T, true>(tw, t * k1));
             Y[(t * A + k1) * C + c] = y;
         });
     }
@@ -84,7 +113,7 @@ __device__ __forceinline__ void c2r_x2_reg(const Space<T> &S, const LaunchEntry 
                 constexpr int k2 = decltype(k2c)::value;
                 cplx<T> val = v[k2];
                 if (k1_0 + c == 0) val.y = T(0);  // c2r uses the real part of the k1 = 0 column
-                out[size_t(t + A * k2) * S.nk1 + k1_0 + c] = val;
+                gstore(out + size_t(t + A * k2) * S.nk1 + k1_0 + c, val);
             });
         }
     }
@@ -108,34 +137,31 @@ __device__ __forceinline__ void c2r_x1_reg(const Space<T> &S, const LaunchEntry 
     const cplx<T> *__restrict__ tw = S.tw + S.p1.tw;
     for (int task = threadIdx.x; task < C * Bq; task += blockDim.x) {
         const int p = task / Bq, t = task - p * Bq;
-        const int xa = 2 * (pair0 + p), xb = xa + 1;
-        const bool live = p < np, hb = xb < N2;
+        const int xa = 2 * (pair0 + min(p, np - 1));  // clamped: loads stay in bounds
+        const bool live = p < np, hb = xa + 1 < N2;
+        const cplx<T> *ra = in + size_t(xa) * nk1, *rb = in + size_t(hb ? xa + 1 : xa) * nk1;
         cplx<T> v[A];
+        // Hermitian completion of the two rows (a + i·b): branch-free selects
+        // after unconditional loads, so all loads are in flight together
         static_for<0, A>([&](auto n1c) {
             constexpr int n1 = decltype(n1c)::value;
             const int k = Bq * n1 + t;
-            cplx<T> z = czero<T>();
-            if (live) {
-                if (k == 0 || (2 * k == N && k <= k1m)) {  // mean and Nyquist: real parts only
-                    const cplx<T> a = in[size_t(xa) * nk1 + k];
-                    z = mk(a.x, hb ? in[size_t(xb) * nk1 + k].x : T(0));
-                } else if (k <= k1m) {
-                    const cplx<T> a = in[size_t(xa) * nk1 + k];
-                    const cplx<T> b = hb ? in[size_t(xb) * nk1 + k] : czero<T>();
-                    z = mk(a.x - b.y, a.y + b.x);
-                } else if (N - k <= k1m) {
-                    const int kk = N - k;
-                    const cplx<T> a = in[size_t(xa) * nk1 + kk];
-                    const cplx<T> b = hb ? in[size_t(xb) * nk1 + kk] : czero<T>();
-                    z = mk(a.x + b.y, b.x - a.y);
-                }
-            }
+            const bool direct = k <= k1m, mirror = !direct && N - k <= k1m;
+            const bool real_only = k == 0 || (2 * k == N && direct);
+            const int kk = direct ? k : (mirror ? N - k : 0);
+            const cplx<T> a = gload(ra + kk);
+            cplx<T> b = gload(rb + kk);
+            if (!hb) b = czero<T>();
+            cplx<T> z = real_only ? mk(a.x, b.x)
+                      : direct   ? mk(a.x - b.y, a.y + b.x)
+                                 : mk(a.x + b.y, b.x - a.y);
+            if (!(live && (direct || mirror))) z = czero<T>();
             v[n1] = z;
         });
         rdft<T, true, A>(v);
         static_for<0, A>([&](auto k1c) {
             constexpr int k1 = decltype(k1c)::value;
-            Y[p * PS + t * RS + k1] = k1 == 0 ? v[0] : cmul(v[k1], twiddle<T, true>(tw, t * k1));
+            Y[p * PS + t * RS + k1] = k1 == 0 ? v[0] : cmul(v[k1], gtwiddle<T, true>(tw, t * k1));
         });
     }
     __syncthreads();
@@ -154,8 +180,8 @@ __device__ __forceinline__ void c2r_x1_reg(const Space<T> &S, const LaunchEntry 
             static_for<0, Bq>([&](auto k2c) {
                 constexpr int k2 = decltype(k2c)::value;
                 const int x = t + A * k2;
-                out[size_t(xa) * N + x] = v[k2].x;
-                if (hb) out[size_t(xa + 1) * N + x] = v[k2].y;
+                gstore(out + size_t(xa) * N + x, v[k2].x);
+                if (hb) gstore(out + size_t(xa + 1) * N + x, v[k2].y);
             });
         }
     }
@@ -180,23 +206,20 @@ __device__ __forceinline__ void r2c_x1_reg(const Space<T> &S, const LaunchEntry 
     const cplx<T> *__restrict__ tw = S.tw + S.p1.tw;
     for (int task = threadIdx.x; task < C * Bq; task += blockDim.x) {
         const int p = task / Bq, t = task - p * Bq;
-        const int xa = 2 * (pair0 + p);
+        const int xa = 2 * (pair0 + min(p, np - 1));  // clamped: loads stay in bounds
         const bool live = p < np, hb = xa + 1 < N2;
+        const T *ra = in + size_t(xa) * N, *rb = in + size_t(hb ? xa + 1 : xa) * N;
         cplx<T> v[A];
         static_for<0, A>([&](auto n1c) {
             constexpr int n1 = decltype(n1c)::value;
             const int x = Bq * n1 + t;
-            cplx<T> z = czero<T>();
-            if (live) {
-                z.x = in[size_t(xa) * N + x];
-                if (hb) z.y = in[size_t(xa + 1) * N + x];
-            }
-            v[n1] = z;
+            const T a = gload(ra + x), b = gload(rb + x);
+            v[n1] = mk(live ? a : T(0), live && hb ? b : T(0));
         });
         rdft<T, false, A>(v);
         static_for<0, A>([&](auto k1c) {
             constexpr int k1 = decltype(k1c)::value;
-            Y[p * PS + t * RS + k1] = k1 == 0 ? v[0] : cmul(v[k1], twiddle<T, false>(tw, t * k1));
+            Y[p * PS + t * RS + k1] = k1 == 0 ? v[0] : cmul(v[k1], gtwiddle<T, false>(tw, t * k1));
         });
     }
     __syncthreads();
@@ -222,8 +245,8 @@ __device__ __forceinline__ void r2c_x1_reg(const Space<T> &S, const LaunchEntry 
         const int xa = 2 * (pair0 + p);
         const cplx<T> zk = Z[p * N + k];
         const cplx<T> zn = Z[p * N + (k == 0 ? 0 : N - k)];
-        out[size_t(xa) * nk1 + k] = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
-        if (xa + 1 < N2) out[size_t(xa + 1) * nk1 + k] = mk(T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x));
+        gstore(out + size_t(xa) * nk1 + k, mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y)));
+        if (xa + 1 < N2) gstore(out + size_t(xa + 1) * nk1 + k, mk(T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x)));
     }
 }
 
@@ -246,38 +269,49 @@ __device__ __forceinline__ void r2c_x2_reg(const Space<T> &S, const LaunchEntry 
     cplx<T> *YH = Y + C * N;
     for (int task = threadIdx.x; task < C * Bq; task += blockDim.x) {
         const int c = task & (C - 1), t = task >> lc;
-        const int k1 = k1_0 + c;
+        const int k1 = k1_0 + min(c, ncols - 1);  // clamped: loads stay in bounds
         const bool live = c < ncols;
-        const T d1 = live ? T(S.D1[k1]) : T(0);
+        const T d1 = T(gload(S.D1 + k1));
         cplx<T> g[A], h[A];
         static_for<0, A>([&](auto n1c) {
             constexpr int n1 = decltype(n1c)::value;
-            const int x2 = Bq * n1 + t;
-            cplx<T> gs = czero<T>(), hs = czero<T>();
-            if (live) {
-                for (int s = 0; s < nG; ++s) {
-                    const int sj = S.src_job0[S.tgt_G[tg][s]] + layer;
-                    const cplx<T> v = S.inter[(size_t(sj) * N + x2) * nk1 + k1];
-                    gs = gs + (S.tgt_Gd1[tg][s] ? imul(d1, v) : v);
-                }
-                for (int s = 0; s < nH; ++s) {
-                    const int sj = S.src_job0[S.tgt_H[tg][s]] + layer;
-                    hs = hs + S.inter[(size_t(sj) * N + x2) * nk1 + k1];
-                }
-            }
-            g[n1] = gs;
-            h[n1] = hs;
+            g[n1] = czero<T>();
+            h[n1] = czero<T>();
         });
+        // one source at a time (uniform loop), its A loads issued together
+        for (int s = 0; s < nG; ++s) {
+            const cplx<T> *src = S.inter + size_t(S.src_job0[S.tgt_G[tg][s]] + layer) * N * nk1 + k1;
+            const bool dx = S.tgt_Gd1[tg][s];
+            static_for<0, A>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                const cplx<T> v = gload(src + size_t(Bq * n1 + t) * nk1);
+                g[n1] = g[n1] + (dx ? imul(d1, v) : v);
+            });
+        }
+        for (int s = 0; s < nH; ++s) {
+            const cplx<T> *src = S.inter + size_t(S.src_job0[S.tgt_H[tg][s]] + layer) * N * nk1 + k1;
+            static_for<0, A>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                h[n1] = h[n1] + gload(src + size_t(Bq * n1 + t) * nk1);
+            });
+        }
+        if (!live) {
+            static_for<0, A>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                g[n1] = czero<T>();
+                h[n1] = czero<T>();
+            });
+        }
         rdft<T, false, A>(g);
         static_for<0, A>([&](auto k1c) {
             constexpr int kk = decltype(k1c)::value;
-            Y[(t * A + kk) * C + c] = kk == 0 ? g[0] : cmul(g[kk], twiddle<T, false>(tw, t * kk));
+            Y[(t * A + kk) * C + c] = kk == 0 ? g[0] : cmul(g[kk], gtwiddle<T, false>(tw, t * kk));
         });
         if (nH > 0) {
             rdft<T, false, A>(h);
             static_for<0, A>([&](auto k1c) {
                 constexpr int kk = decltype(k1c)::value;
-                YH[(t * A + kk) * C + c] = kk == 0 ? h[0] : cmul(h[kk], twiddle<T, false>(tw, t * kk));
+                YH[(t * A + kk) * C + c] = kk == 0 ? h[0] : cmul(h[kk], gtwiddle<T, false>(tw, t * kk));
             });
         }
     }
@@ -310,10 +344,10 @@ __device__ __forceinline__ void r2c_x2_reg(const Space<T> &S, const LaunchEntry 
                     i2 = S.K2 - (N - pos);
                 if (i2 >= 0) {
                     cplx<T> v = g[k2];
-                    if (nH > 0) v = v + imul(T(S.D2[i2]), h[k2]);
+                    if (nH > 0) v = v + imul(T(gload(S.D2 + i2)), h[k2]);
                     const size_t o = size_t(i2) * S.K1 + k1_0 + c;
                     if (S.xfer_centered) v = cmul(v, centre_shift<T>(S, k1_0 + c, i2, -1.0));
-                    out[o] = out[o] + scale * v;
+                    gstore(out + o, gload(out + o) + scale * v);
                 }
             });
         }
