@@ -784,10 +784,10 @@ class Engine : public EngineBase {
             LaunchEntry E{};
             E.space = int(i);
             E.block_begin = lin_pass.nblocks;
-            E.blocks_per_job = int((size_t(I.nk) * (3 * size_t(I.n3) - 1) + 255) / 256);
-            E.njobs = 1;
+            E.blocks_per_job = int((size_t(I.nk) * size_t(I.n3) + 255) / 256);  // one job per field
+            E.njobs = 3;
             lin_pass.ent.push_back(E);
-            lin_pass.nblocks += E.blocks_per_job;
+            lin_pass.nblocks += E.blocks_per_job * E.njobs;
         }
         for (int k = 0; k < 5; ++k) finish(pp[k]);
         finish(lin_pass);

@@ -319,31 +319,35 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-    const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    if (gid >= P * S.n3) return;
-    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
-    const size_t x = gid - size_t(l) * P;
+    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
+    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
+    const int n3 = S.n3;
+    if (gid >= P * uint32_t(n3)) return;
+    const int l = int(gid / P);
+    const uint32_t x = gid - uint32_t(l) * P;
     const T *u = S.in_phys[0], *v = S.in_phys[1], *w = S.in_phys[2];
     const T *o1 = S.in_phys[3], *o2 = S.in_phys[4], *o3 = S.in_phys[5];
     T *a1 = S.src_phys[0], *a2 = S.src_phys[1], *a3 = S.src_phys[2];
-    const size_t i = gid;
-    const T ul = u[i], vl = v[i], o3l = o3[i];
+    // all loads first, at clamped levels: I-levels l-1 and l, C-level l+1
+    const uint32_t i = gid;
+    const uint32_t jb = uint32_t(max(l - 1, 0)) * P + x, ja = uint32_t(min(l, n3 - 2)) * P + x;
+    const uint32_t in = uint32_t(min(l + 1, n3 - 1)) * P + x;
+    const T ul = gload(u + i), vl = gload(v + i), o3l = gload(o3 + i);
+    const T o2b = gload(o2 + jb), o1b = gload(o1 + jb), wb = gload(w + jb);
+    const T o2a = gload(o2 + ja), o1a = gload(o1 + ja), wa = gload(w + ja);
+    const T un = gload(u + in), vn = gload(v + in);
     T p2b = T(0), p2a = T(0), p1b = T(0), p1a = T(0);
     if (l > 0) {
-        const size_t j = size_t(l - 1) * P + x;
-        p2b = o2[j] * w[j];
-        p1b = o1[j] * w[j];
+        p2b = o2b * wb;
+        p1b = o1b * wb;
     }
-    if (l < S.n3 - 1) {
-        const size_t j = size_t(l) * P + x;
-        p2a = o2[j] * w[j];
-        p1a = o1[j] * w[j];
-        const T un = u[j + P], vn = v[j + P];
-        a3[j] = o2[j] * T(0.5) * (ul + un) - o1[j] * T(0.5) * (vl + vn);
+    if (l < n3 - 1) {
+        p2a = o2a * wa;
+        p1a = o1a * wa;
+        gstore(a3 + ja, o2a * T(0.5) * (ul + un) - o1a * T(0.5) * (vl + vn));
     }
-    a1[i] = o3l * vl - T(0.5) * (p2b + p2a);
-    a2[i] = T(0.5) * (p1b + p1a) - o3l * ul;
+    gstore(a1 + i, o3l * vl - T(0.5) * (p2b + p2a));
+    gstore(a2 + i, T(0.5) * (p1b + p1a) - o3l * ul);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,54 +356,74 @@ __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict
 // sources: 0 R1 (vel1), 1 R2 (vel2), 2 R3 (vel3), 3 T11 (vel1_1), 4 T22 (vel2_2),
 //          5 T12 (vel1_2 & vel2_1), 6 T13 (vel3_1), 7 T23 (vel3_2)
 // ---------------------------------------------------------------------------
+// The vertical stencil of one point (C-level l, I-levels l-1 and l), loaded
+// together at clamped levels before any value is used; the strain-rate
+// components are then formed from registers.
 template <typename T>
 struct SgsPoint {
     const Space<T> &S;
-    size_t P, x;
-    __device__ SgsPoint(const Space<T> &s, size_t p, size_t xx) : S(s), P(p), x(xx) {}
-    __device__ T f(int t, int l) const { return S.in_phys[t][size_t(l) * P + x]; }
-    // S33 on C-layer l (vel3_3 via dx3_i2c!, physical Dirichlet values)
-    __device__ T s33(int l) const {
-        const T a = T(S.alpha_c[l]);
-        const T wb = l > 0 ? f(2, l - 1) : T(S.wbc[0]);
-        const T wa = l < S.n3 - 1 ? f(2, l) : T(S.wbc[1]);
-        return (-a) * wb + a * wa;
+    int l, n3;
+    T u[3], v[3];           // C-levels l-1, l, l+1
+    T w[3];                 // I-levels l-1, l, l+1
+    T u1[2], v2[2], u2[2], v1[2];  // C-levels l, l+1
+    T w1[2], w2[2];         // I-levels l-1, l
+    T ac[2], ai[2];         // alpha_c[l], alpha_c[l+1]; alpha_i[l], alpha_i[l+1]
+    __device__ SgsPoint(const Space<T> &s, uint32_t P, uint32_t x, int lev) : S(s), l(lev), n3(s.n3) {
+        const uint32_t cm = uint32_t(max(l - 1, 0)) * P + x, c0 = uint32_t(l) * P + x;
+        const uint32_t cp = uint32_t(min(l + 1, n3 - 1)) * P + x;
+        const uint32_t i0 = uint32_t(min(l, n3 - 2)) * P + x, ip = uint32_t(min(l + 1, n3 - 2)) * P + x;
+        const T *const *f = S.in_phys;
+        u[0] = gload(f[0] + cm), u[1] = gload(f[0] + c0), u[2] = gload(f[0] + cp);
+        v[0] = gload(f[1] + cm), v[1] = gload(f[1] + c0), v[2] = gload(f[1] + cp);
+        w[0] = gload(f[2] + cm), w[1] = gload(f[2] + i0), w[2] = gload(f[2] + ip);
+        u1[0] = gload(f[3] + c0), u1[1] = gload(f[3] + cp);
+        v2[0] = gload(f[4] + c0), v2[1] = gload(f[4] + cp);
+        u2[0] = gload(f[5] + c0), u2[1] = gload(f[5] + cp);
+        v1[0] = gload(f[6] + c0), v1[1] = gload(f[6] + cp);
+        w1[0] = gload(f[7] + cm), w1[1] = gload(f[7] + i0);
+        w2[0] = gload(f[8] + cm), w2[1] = gload(f[8] + i0);
+        ac[0] = T(gload(S.alpha_c + l)), ac[1] = T(gload(S.alpha_c + min(l + 1, n3 - 1)));
+        ai[0] = T(gload(S.alpha_i + l)), ai[1] = T(gload(S.alpha_i + l + 1));
     }
-    __device__ T s12(int l) const { return (f(5, l) + f(6, l)) / T(2); }
-    // S13, S23 on I-layer j (vel1_3 via dx3_c2i!)
-    __device__ T s13(int j) const {
-        const T a = T(S.alpha_i[j + 1]);
-        return ((-a) * f(0, j) + a * f(0, j + 1) + f(7, j)) / T(2);
+    // S33 on C-layer l (k = 0) or l+1 (k = 1) (vel3_3 via dx3_i2c!, physical Dirichlet values)
+    __device__ T s33(int k) const {
+        const int lk = l + k;
+        const T wb = lk > 0 ? w[k] : T(S.wbc[0]);
+        const T wa = lk < n3 - 1 ? w[k + 1] : T(S.wbc[1]);
+        return (-ac[k]) * wb + ac[k] * wa;
     }
-    __device__ T s23(int j) const {
-        const T a = T(S.alpha_i[j + 1]);
-        return ((-a) * f(1, j) + a * f(1, j + 1) + f(8, j)) / T(2);
-    }
+    __device__ T s12(int k) const { return (u2[k] + v1[k]) / T(2); }
+    // S13, S23 on I-layer l-1 (k = 0) or l (k = 1) (vel1_3 via dx3_c2i!)
+    __device__ T s13(int k) const { return ((-ai[k]) * u[k] + ai[k] * u[k + 1] + w1[k]) / T(2); }
+    __device__ T s23(int k) const { return ((-ai[k]) * v[k] + ai[k] * v[k + 1] + w2[k]) / T(2); }
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-    const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    if (gid >= P * S.n3) return;
-    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
-    const size_t x = gid - size_t(l) * P;
-    SgsPoint<T> q(S, P, x);
+    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
+    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
+    if (gid >= P * uint32_t(S.n3)) return;
+    const int l = int(gid / P);
+    const uint32_t x = gid - uint32_t(l) * P;
     const int n3 = S.n3;
+    const T lcl = T(gload(S.lc + l)), lil = T(gload(S.li + min(l, n3 - 2)));
+    const SgsPoint<T> q(S, P, x, l);
     // C-nodes: total_strain! (:208, :225-229)
-    const T S11 = q.f(3, l), S22 = q.f(4, l), S33 = q.s33(l), S12 = q.s12(l);
+    const T S11 = q.u1[0], S22 = q.v2[0], S33 = q.s33(0), S12 = q.s12(0);
     T sc = S11 * S11 + S22 * S22 + S33 * S33 + T(2) * S12 * S12;
+#pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
         // lower neighbour (I-layer l-1 or wall), upper neighbour (I-layer l or wall)
-        const T sb = l > 0 ? (comp == 0 ? q.s13(l - 1) : q.s23(l - 1)) : T(0);
-        const T sa = l < n3 - 1 ? (comp == 0 ? q.s13(l) : q.s23(l)) : T(0);
+        const T sb = l > 0 ? (comp == 0 ? q.s13(0) : q.s23(0)) : T(0);
+        const T sa = l < n3 - 1 ? (comp == 0 ? q.s13(1) : q.s23(1)) : T(0);
+        const T ul = comp == 0 ? q.u[1] : q.v[1];
         if (l == 0 && S.wall_rough[0]) {
-            const T val = q.f(comp, 0) * T(S.fstrain[0]);
+            const T val = ul * T(S.fstrain[0]);
             sc += T(2) * (val * val);
         } else if (l == n3 - 1 && S.wall_rough[1]) {
-            const T val = q.f(comp, n3 - 1) * T(S.fstrain[1]);
+            const T val = ul * T(S.fstrain[1]);
             sc += T(2) * (val * val);
         } else {
             const T s = sb + sa;
@@ -407,24 +431,22 @@ __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restric
         }
     }
     sc = sqrt(T(2) * sc);
-    const T lcl = T(S.lc[l]);
-    S.visc_c[gid] = sc * (lcl * lcl);
+    gstore(S.visc_c + gid, sc * (lcl * lcl));
     // I-nodes (:209-220)
     if (l < n3 - 1) {
-        const T a13 = q.s13(l), a23 = q.s23(l);
+        const T a13 = q.s13(1), a23 = q.s23(1);
         T si = T(2) * a13 * a13 + T(2) * a23 * a23;
         T s;
-        s = q.f(3, l) + q.f(3, l + 1);
+        s = q.u1[0] + q.u1[1];
         si += s * s / T(4);
-        s = q.f(4, l) + q.f(4, l + 1);
+        s = q.v2[0] + q.v2[1];
         si += s * s / T(4);
-        s = S33 + q.s33(l + 1);
+        s = S33 + q.s33(1);
         si += s * s / T(4);
-        s = S12 + q.s12(l + 1);
+        s = S12 + q.s12(1);
         si += T(2) * (s * s) / T(4);
         si = sqrt(T(2) * si);
-        const T lil = T(S.li[l]);
-        S.visc_i[size_t(l) * P + x] = si * (lil * lil);
+        gstore(S.visc_i + gid, si * (lil * lil));
     }
 }
 
@@ -432,51 +454,53 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
-    const size_t P = size_t(S.N1) * S.N2;
-    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    if (gid >= P * S.n3) return;
-    const int l = int(uint32_t(gid) / uint32_t(P));  // point counts < 2^31 (checked at set-up)
-    const size_t x = gid - size_t(l) * P;
-    SgsPoint<T> q(S, P, x);
+    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
+    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
+    if (gid >= P * uint32_t(S.n3)) return;
+    const int l = int(gid / P);
+    const uint32_t x = gid - uint32_t(l) * P;
     const int n3 = S.n3;
-    const T evc = S.visc_c[gid];
+    const T evc = gload(S.visc_c + gid);
+    const T evn = gload(S.visc_c + uint32_t(min(l + 1, n3 - 1)) * P + x);
+    const T evb = gload(S.visc_i + uint32_t(max(l - 1, 0)) * P + x);
+    const T eva = gload(S.visc_i + uint32_t(min(l, n3 - 2)) * P + x);
+    const SgsPoint<T> q(S, P, x, l);
     // τ11, τ22, τ12 on C-nodes: rates vel1_1 -= τ11, vel2_2 -= τ22, vel1_2/vel2_1 -= τ12
-    S.src_phys[3][gid] = -(T(-2) * evc * q.f(3, l));
-    S.src_phys[4][gid] = -(T(-2) * evc * q.f(4, l));
-    S.src_phys[5][gid] = -(T(-2) * evc * q.s12(l));
+    gstore(S.src_phys[3] + gid, -(T(-2) * evc * q.u1[0]));
+    gstore(S.src_phys[4] + gid, -(T(-2) * evc * q.v2[0]));
+    gstore(S.src_phys[5] + gid, -(T(-2) * evc * q.s12(0)));
     // vel3 -= d/dx3 τ33 (dx3_c2i!) on I-layer l
     if (l < n3 - 1) {
-        const T a = T(S.alpha_i[l + 1]);
-        const T t0 = T(-2) * evc * q.s33(l);
-        const T t1 = T(-2) * S.visc_c[gid + P] * q.s33(l + 1);
-        S.src_phys[2][size_t(l) * P + x] = -((-a) * t0 + a * t1);
-        const T evi = S.visc_i[size_t(l) * P + x];
-        S.src_phys[6][size_t(l) * P + x] = -(T(-2) * evi * q.s13(l));
-        S.src_phys[7][size_t(l) * P + x] = -(T(-2) * evi * q.s23(l));
+        const T a = q.ai[1];
+        const T t0 = T(-2) * evc * q.s33(0);
+        const T t1 = T(-2) * evn * q.s33(1);
+        gstore(S.src_phys[2] + gid, -((-a) * t0 + a * t1));
+        gstore(S.src_phys[6] + gid, -(T(-2) * eva * q.s13(1)));
+        gstore(S.src_phys[7] + gid, -(T(-2) * eva * q.s23(1)));
     }
     // vel1 -= d/dx3 τ13, vel2 -= d/dx3 τ23 on C-layer l (dx3_i2c! with wall stress)
-    const T ac = T(S.alpha_c[l]);
+    const T ac = q.ac[0];
+#pragma unroll
     for (int comp = 0; comp < 2; ++comp) {
         T tb, ta;
+        const T ul = comp == 0 ? q.u[1] : q.v[1];
         if (l > 0) {
-            const T evi = S.visc_i[size_t(l - 1) * P + x];
-            tb = T(-2) * evi * (comp == 0 ? q.s13(l - 1) : q.s23(l - 1));
+            tb = T(-2) * evb * (comp == 0 ? q.s13(0) : q.s23(0));
         } else if (S.wall_rough[0]) {
-            const T u0 = q.f(0, 0), v0 = q.f(1, 0);
-            tb = T(S.fstress[0]) * q.f(comp, 0) * sqrt(u0 * u0 + v0 * v0);
+            const T u0 = q.u[1], v0 = q.v[1];
+            tb = T(S.fstress[0]) * ul * sqrt(u0 * u0 + v0 * v0);
         } else {
             tb = T(0);
         }
         if (l < n3 - 1) {
-            const T evi = S.visc_i[size_t(l) * P + x];
-            ta = T(-2) * evi * (comp == 0 ? q.s13(l) : q.s23(l));
+            ta = T(-2) * eva * (comp == 0 ? q.s13(1) : q.s23(1));
         } else if (S.wall_rough[1]) {
-            const T u1 = q.f(0, n3 - 1), v1 = q.f(1, n3 - 1);
-            ta = T(S.fstress[1]) * q.f(comp, n3 - 1) * sqrt(u1 * u1 + v1 * v1);
+            const T u1 = q.u[1], v1 = q.v[1];
+            ta = T(S.fstress[1]) * ul * sqrt(u1 * u1 + v1 * v1);
         } else {
             ta = T(0);
         }
-        S.src_phys[comp][gid] = -((-ac) * tb + ac * ta);
+        gstore(S.src_phys[comp] + gid, -((-ac) * tb + ac * ta));
     }
 }
 
@@ -525,24 +549,14 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const LinDesc<T> &L = descs[E.space];
     const int nk = L.nk, n3 = L.n3;
-    const size_t per_c = size_t(nk) * n3, per_i = size_t(nk) * (n3 - 1);
-    const size_t gid = size_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    int f;
-    size_t r;
-    if (gid < per_c) {
-        f = 0;
-        r = gid;
-    } else if (gid < 2 * per_c) {
-        f = 1;
-        r = gid - per_c;
-    } else if (gid < 2 * per_c + per_i) {
-        f = 2;
-        r = gid - 2 * per_c;
-    } else {
-        return;
-    }
-    const int l = int(uint32_t(r) / uint32_t(nk)), m = int(r - size_t(l) * nk);
+    // one job per field: the field index is uniform over the block, so the
+    // descriptor fields it selects are scalar loads
+    const int bj = blockIdx.x - E.block_begin;
+    const int f = bj / E.blocks_per_job;
     const int nl = f < 2 ? n3 : n3 - 1;
+    const uint32_t r = uint32_t(bj - f * E.blocks_per_job) * blockDim.x + threadIdx.x;
+    if (r >= uint32_t(nl) * uint32_t(nk)) return;
+    const int l = int(r / uint32_t(nk)), m = int(r - uint32_t(l) * nk);
     const cplx<T> *q = L.state[sel][f];
     cplx<T> *du = L.du[f];
     const bool mixed00 = (m == 0) && (L.shadow[sel][f] != nullptr);
@@ -567,25 +581,39 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
         du[size_t(l) * nk] = mk(T(0), T(0));
         return;
     }
+    const bool diff = L.has_diff[f] != 0;
+    const T src = T(L.src[f]);
+    const size_t i = size_t(l) * nk + m;
     cplx<T> res = mk(T(0), T(0));
-    if (L.has_diff[f]) {
-        const size_t i = size_t(l) * nk + m;
-        const cplx<T> c = q[i];
+    if (diff) {
+        // the column values a thread needs, loaded together with clamped
+        // levels: below, at and above l, and the second level from the
+        // boundary for the ghost layers
+        const int llo = max(l - 1, 0), lhi = min(l + 1, nl - 1), lx = l == 0 ? min(2, nl - 1) : max(l - 2, 0);
+        const cplx<T> c = gload(q + i);
+        const cplx<T> qlo = gload(q + size_t(llo) * nk + m);
+        const cplx<T> qhi = gload(q + size_t(lhi) * nk + m);
+        const cplx<T> qx = gload(q + size_t(lx) * nk + m);
+        const int k1 = m % L.K1, i2 = m / L.K1;
+        const T dd = T(gload(L.DD1 + k1)) + T(gload(L.DD2 + i2));
+        const double *pa = f < 2 ? L.alpha_i : L.alpha_c, *pb = f < 2 ? L.alpha_c : L.alpha_i;
+        const T a0 = T(gload(pa + l)), a1 = T(gload(pb + l + (f < 2 ? 0 : 1))), a2 = T(gload(pa + l + 1));
+        const T nu = T(L.nu[f]);
         cplx<T> lo, hi;
         if (f < 2) {
-            lo = l > 0 ? q[i - nk]
-                       : ghost<T>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], m == 0, false, q[m], q[nk + m], q[2 * nk + m]);
-            hi = l < nl - 1 ? q[i + nk]
-                            : ghost<T>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], m == 0, true, q[i], q[i - nk], q[i - 2 * nk]);
+            lo = l > 0 ? qlo : ghost<T>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], m == 0, false, c, qhi, qx);
+            hi = l < nl - 1 ? qhi : ghost<T>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], m == 0, true, c, qlo, qx);
         } else {
             // I-nodes: Dirichlet values enter every mode (boundary_conditions.jl:263-274)
-            lo = l > 0 ? q[i - nk] : mk(T(L.bc_val[2][0]), T(0));
-            hi = l < nl - 1 ? q[i + nk] : mk(T(L.bc_val[2][1]), T(0));
+            lo = l > 0 ? qlo : mk(T(L.bc_val[2][0]), T(0));
+            hi = l < nl - 1 ? qhi : mk(T(L.bc_val[2][1]), T(0));
         }
-        res = laplacian_point<T, T>(L, f, l, m, c, lo, hi);
+        const T c0 = a0 * a1, c2 = a1 * a2;
+        const T c1 = dd - a0 * a1 - a1 * a2;
+        res = mk(nu * (c0 * lo.x + c1 * c.x + c2 * hi.x), nu * (c0 * lo.y + c1 * c.y + c2 * hi.y));
     }
-    if (m == 0) res.x += T(L.src[f]);
-    du[size_t(l) * nk + m] = res;
+    if (m == 0) res.x += src;
+    gstore(du + i, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -764,58 +792,85 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     const int l0 = ty * CL;
     const int mm = active ? m : 0;
     cplx<T> *const *outp = stage_writes_ut(stage) ? D.ut : D.u;
-    bool bad = false;
+    const bool rd_ut = stage_reads_ut(stage), rd_du = stage > 0, sv_du = stage_saves_du(stage);
+    // Every load of the chunk is issued before any value is used: indices are
+    // clamped into the arrays (vel3 has n3-1 levels) and values outside the
+    // column are replaced by zero afterwards, so the loads carry no branches.
     cplx<T> c1[CL], c2[CL], c3[CL];
+    double gam[CL], ac[CL], ai[CL + 1];
+    {
+        cplx<T> lu[3][CL], lt[3][CL], ld[3][CL];
+        cplx<float> le[3][CL];
+        const cplx<T> *const *tsrc = rd_ut ? D.ut : D.u;
+        const cplx<T> *const *dsrc = rd_du ? D.du : D.u;
 #pragma unroll
-    for (int j = 0; j < CL; ++j) {
-        const int l = l0 + j;
-        c1[j] = c2[j] = c3[j] = mk(T(0), T(0));
-        if (active && l < n3) {
+        for (int j = 0; j < CL; ++j) {
+            const int lc = min(l0 + j, n3 - 1), li = min(l0 + j, n3 - 2);
+#pragma unroll
             for (int f = 0; f < 3; ++f) {
-                if (f == 2 && l == n3 - 1) break;
-                const size_t i = size_t(l) * nk + m;
-                cplx<T> u = D.u[f][i], ut = mk(T(0), T(0)), du = mk(T(0), T(0));
-                if (stage_reads_ut(stage)) ut = D.ut[f][i];
-                if (stage > 0) {
-                    du = D.du[f][i];
-                    if (D.ext_stride) {
-                        const cplx<float> e = D.du_add[f][size_t(l) * D.ext_stride];
-                        du = mk(du.x + T(e.x), du.y + T(e.y));
-                    }
-                    if (!(isfinite(du.x) && isfinite(du.y))) bad = true;
+                const size_t i = size_t(f == 2 ? li : lc) * nk + mm;
+                lu[f][j] = gload(D.u[f] + i);
+                lt[f][j] = gload(tsrc[f] + i);
+                ld[f][j] = gload(dsrc[f] + i);
+                if (D.ext_stride) le[f][j] = gload(D.du_add[f] + size_t(f == 2 ? li : lc) * D.ext_stride);
+            }
+            gam[j] = D.has_pressure ? gload(D.gamma + size_t(lc) * nk + mm) : 1.0;
+            ac[j] = gload(D.alpha_c + lc);
+            ai[j] = gload(D.alpha_i + lc);
+        }
+        ai[CL] = gload(D.alpha_i + min(l0 + CL, n3));
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < CL; ++j) {
+            const int l = l0 + j;
+            const bool ok = active && l < n3;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const bool okf = ok && (f < 2 || l < n3 - 1);
+                cplx<T> du = mk(T(0), T(0)), ut = mk(T(0), T(0));
+                if (rd_ut) ut = lt[f][j];
+                if (rd_du) {
+                    du = ld[f][j];
+                    if (D.ext_stride) du = mk(du.x + T(le[f][j].x), du.y + T(le[f][j].y));
+                    if (okf && !(isfinite(du.x) && isfinite(du.y))) bad = true;
                 }
-                const cplx<T> c = combine<T, T>(stage, u, ut, du, T(dt));
-                if (stage_saves_du(stage)) D.ut[f][i] = du;  // Adams–Bashforth: keep f(u) for the next step
+                cplx<T> c = combine<T, T>(stage, lu[f][j], ut, du, T(dt));
+                if (sv_du && okf) {  // Adams–Bashforth: keep f(u) for the next step
+                    const size_t i = size_t(l) * nk + m;
+                    gstore(D.ut[f] + i, du);
+                }
+                if (!okf) c = mk(T(0), T(0));
                 if (f == 0) c1[j] = c;
                 else if (f == 1) c2[j] = c;
                 else c3[j] = c;
             }
         }
+        if (bad) *D.flag = 1;
     }
-    if (bad) *D.flag = 1;
     if (!D.has_pressure) {
         if (stage > 0 && active) {
+            cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
 #pragma unroll
             for (int j = 0; j < CL; ++j) {
                 const int l = l0 + j;
-                if (l >= n3) break;
-                const size_t i = size_t(l) * nk + m;
-                outp[0][i] = c1[j];
-                outp[1][i] = c2[j];
-                if (l < n3 - 1) outp[2][i] = c3[j];
-                if (D.ext_stride) {
-                    cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
-                    const size_t e = size_t(l) * D.ext_stride;
-                    mir[0][e] = mk(float(c1[j].x), float(c1[j].y));
-                    mir[1][e] = mk(float(c2[j].x), float(c2[j].y));
-                    if (l < n3 - 1) mir[2][e] = mk(float(c3[j].x), float(c3[j].y));
+                if (l < n3) {
+                    const size_t i = size_t(l) * nk + m;
+                    gstore(outp[0] + i, c1[j]);
+                    gstore(outp[1] + i, c2[j]);
+                    if (l < n3 - 1) gstore(outp[2] + i, c3[j]);
+                    if (D.ext_stride) {
+                        const size_t e = size_t(l) * D.ext_stride;
+                        gstore(mir[0] + e, mk(float(c1[j].x), float(c1[j].y)));
+                        gstore(mir[1] + e, mk(float(c2[j].x), float(c2[j].y)));
+                        if (l < n3 - 1) gstore(mir[2] + e, mk(float(c3[j].x), float(c3[j].y)));
+                    }
                 }
             }
         }
         return;
     }
     const int k1 = mm % D.K1, i2 = mm / D.K1;
-    const double d1 = D.D1[k1], d2 = D.D2[i2];
+    const double d1 = gload(D.D1 + k1), d2 = gload(D.D2 + i2);
     // w at the I-level below this chunk (last level of the previous chunk)
     {
         const int jl = min(CL, n3 - l0) - 1;
@@ -846,7 +901,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
             const cplx<double> a1 = mk(double(c1[j].x), double(c1[j].y));
             const cplx<double> a2 = mk(double(c2[j].x), double(c2[j].y));
             const cplx<double> a3 = mk(double(c3[j].x), double(c3[j].y));
-            const double hf = 1.0 / D.alpha_c[l];
+            const double hf = 1.0 / ac[j];
             cplx<double> r = mk((-d1 * a1.y) * hf + (-d2 * a2.y) * hf, (d1 * a1.x) * hf + (d2 * a2.x) * hf);
             if (l > 0) r = mk(r.x - wb.x, r.y - wb.y);
             if (l < n3 - 1) r = mk(r.x + a3.x, r.y + a3.y);
@@ -855,13 +910,13 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
                 if (l == n3 - 1) r.x += D.wbc[1];
                 if (l == n3 - 1) r = mk(0.0, 0.0);
             }
-            const double gl = active ? D.gamma[size_t(l) * nk + m] : 1.0;
+            const double gl = active ? gam[j] : 1.0;
             g[j] = gl;
             if (l == 0) {
                 y[j] = mk(gl * r.x, gl * r.y);
                 P[j] = 0.0;
             } else {
-                const double b = D.alpha_i[l];
+                const double b = ai[j];
                 y[j] = mk(gl * (r.x - b * yprev.x), gl * (r.y - b * yprev.y));
                 P[j] = -(gl * b) * Pprev;
             }
@@ -894,7 +949,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
             if (l == n3 - 1) {
                 P[j] = 0.0;  // Q
             } else {
-                const double gb = g[j] * D.alpha_i[l + 1];
+                const double gb = g[j] * ai[j + 1];
                 y[j] = mk(y[j].x - gb * pnext.x, y[j].y - gb * pnext.y);
                 P[j] = -gb * Qnext;
             }
@@ -917,32 +972,33 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     }
     if (!active) return;
     // gradient (add_gradient!, pressure.jl:405-426) and store
+    cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
 #pragma unroll
     for (int j = 0; j < CL; ++j) {
         const int l = l0 + j;
-        if (l >= n3) break;
-        const size_t i = size_t(l) * nk + m;
-        const cplx<double> p = y[j];
-        cplx<double> v1 = mk(double(c1[j].x), double(c1[j].y));
-        cplx<double> v2 = mk(double(c2[j].x), double(c2[j].y));
-        v1 = mk(v1.x + d1 * p.y, v1.y - d1 * p.x);
-        v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
-        outp[0][i] = mk(T(v1.x), T(v1.y));
-        outp[1][i] = mk(T(v2.x), T(v2.y));
-        cplx<float> *const *mir = stage_writes_ut(stage) ? D.mirror_ut : D.mirror_u;
-        const size_t e = size_t(l) * D.ext_stride;
-        if (D.ext_stride) {
-            mir[0][e] = mk(float(v1.x), float(v1.y));
-            mir[1][e] = mk(float(v2.x), float(v2.y));
-        }
-        if (l < n3 - 1) {
-            const cplx<double> pn = (j + 1 < CL) ? y[j + 1] : pin;
-            const double a = D.alpha_i[l + 1];
-            const cplx<double> gr = mk((-a) * p.x + a * pn.x, (-a) * p.y + a * pn.y);
-            cplx<double> v3 = mk(double(c3[j].x), double(c3[j].y));
-            v3 = mk(v3.x - gr.x, v3.y - gr.y);
-            outp[2][i] = mk(T(v3.x), T(v3.y));
-            if (D.ext_stride) mir[2][e] = mk(float(v3.x), float(v3.y));
+        if (l < n3) {
+            const size_t i = size_t(l) * nk + m;
+            const cplx<double> p = y[j];
+            cplx<double> v1 = mk(double(c1[j].x), double(c1[j].y));
+            cplx<double> v2 = mk(double(c2[j].x), double(c2[j].y));
+            v1 = mk(v1.x + d1 * p.y, v1.y - d1 * p.x);
+            v2 = mk(v2.x + d2 * p.y, v2.y - d2 * p.x);
+            gstore(outp[0] + i, mk(T(v1.x), T(v1.y)));
+            gstore(outp[1] + i, mk(T(v2.x), T(v2.y)));
+            const size_t e = size_t(l) * D.ext_stride;
+            if (D.ext_stride) {
+                gstore(mir[0] + e, mk(float(v1.x), float(v1.y)));
+                gstore(mir[1] + e, mk(float(v2.x), float(v2.y)));
+            }
+            if (l < n3 - 1) {
+                const cplx<double> pn = (j + 1 < CL) ? y[j + 1] : pin;
+                const double a = ai[j + 1];
+                const cplx<double> gr = mk((-a) * p.x + a * pn.x, (-a) * p.y + a * pn.y);
+                cplx<double> v3 = mk(double(c3[j].x), double(c3[j].y));
+                v3 = mk(v3.x - gr.x, v3.y - gr.y);
+                gstore(outp[2] + i, mk(T(v3.x), T(v3.y)));
+                if (D.ext_stride) gstore(mir[2] + e, mk(float(v3.x), float(v3.y)));
+            }
         }
     }
 }
