@@ -12,7 +12,7 @@
 #include "bld.h"
 #include "kernels.hpp"
 #include "mlp.hpp"
-#include "transforms_reg.hpp"
+#include "transforms_2d.hpp"
 
 namespace bld {
 
@@ -161,7 +161,7 @@ struct EngineBase {
 
 enum KernelKind {
     KK_LINEAR = 0, KK_C2R_X2, KK_C2R_X1, KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_R2C_X1, KK_R2C_X2,
-    KK_STAGE, KK_CHECK, KK_SGS_MLP, KK_COUNT
+    KK_STAGE, KK_CHECK, KK_SGS_MLP, KK_C2R_2D, KK_COUNT
 };
 
 // One model (one bld_config): its device state, constants and descriptors.
@@ -295,6 +295,8 @@ class Engine : public EngineBase {
     // r2c_x2 of the SGS spaces is separate (it accumulates into the same rates)
     std::vector<Pass> tp[4];
     std::vector<Pass> tp_r2c_x2_sgs;
+    std::vector<Pass> tp_c2r_2d;  // fused c2r of whole planes (square grids that fit in LDS)
+    bool fused_c2r = true;
     std::vector<std::pair<int, int>> tp_entry_space[5];  // (space, batch) per pass, for bld_tuning
     Pass pp[5];  // physics: advection products, SGS viscosity, SGS stress, learned stress, its rates
     Pass lin_pass, stage_pass, stage00_pass, xfer[4];
@@ -670,8 +672,28 @@ class Engine : public EngineBase {
         return p2;
     }
 
+    // the fused c2r: square grid, even N with a register FFT, plane in LDS
+    static bool c2r_2d_fits(const Space<T> &S) {
+        const int N = S.N2;
+        if (S.N1 != N || !c2r2d_supported<T>(N)) return false;
+        return c2r2d_lds(N, S.nk1, sizeof(cplx<T>)) <= 160 * 1024;
+    }
+
     void add_entry(Pass &ps, int kind, int sidx, int batch = 0) {
         const Space<T> &S = spaces[sidx];
+        if (kind == PASS_C2R_2D) {  // one block per (term, layer) plane
+            LaunchEntry E{};
+            E.space = sidx;
+            E.batch = 1;
+            E.blocks_per_job = 1;
+            E.njobs = S.in_job0[S.nin];
+            E.block_begin = ps.nblocks;
+            ps.ent.push_back(E);
+            ps.kind.push_back(kind);
+            ps.nblocks += E.njobs;
+            ps.smem = std::max(ps.smem, c2r2d_lds(S.N2, S.nk1, sizeof(cplx<T>)));
+            return;
+        }
         int n, bufs, avail, njobs;
         pass_geometry(kind, S, n, bufs, avail, njobs);
         LaunchEntry E{};
@@ -728,6 +750,7 @@ class Engine : public EngineBase {
     void build_transform_passes(const std::vector<std::vector<int>> *batches = nullptr) {
         for (int k = 0; k < 4; ++k) tp[k].clear();
         tp_r2c_x2_sgs.clear();
+        tp_c2r_2d.clear();
         for (int k = 0; k < 5; ++k) tp_entry_space[k].clear();
         std::vector<int> model_spaces;
         for (auto &I : inst)
@@ -736,7 +759,12 @@ class Engine : public EngineBase {
         for (size_t i = 0; i < model_spaces.size(); ++i) {
             const int s = model_spaces[i];
             auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
-            for (int k = 0; k < 3; ++k) bucket_add(tp[k], k, s, b(k));
+            if (fused_c2r && c2r_2d_fits(spaces[s])) {
+                bucket_add(tp_c2r_2d, PASS_C2R_2D, s, 0);
+                bucket_add(tp[PASS_R2C_X1], PASS_R2C_X1, s, b(PASS_R2C_X1));
+            } else {
+                for (int k = 0; k < 3; ++k) bucket_add(tp[k], k, s, b(k));
+            }
             if (spaces[s].kind == SK_ADVECTION)
                 bucket_add(tp[PASS_R2C_X2], PASS_R2C_X2, s, b(3));
             else
@@ -749,6 +777,8 @@ class Engine : public EngineBase {
         }
         for (int k = 0; k < 4; ++k) bucket_finish(tp[k]);
         bucket_finish(tp_r2c_x2_sgs);
+        bucket_finish(tp_c2r_2d);
+        if (!tp_c2r_2d.empty()) HIPCHECK(allow_large_lds<T>());
     }
 
     void build_plans() {
@@ -890,7 +920,8 @@ class Engine : public EngineBase {
     // join before the r2c passes (which read the products and accumulate into
     // du).  With per-kernel profiling everything stays on the engine stream.
     void rhs(int sel) {
-        const bool side = !profiling && !tp[PASS_C2R_X2].empty();
+        const bool any_c2r = !tp[PASS_C2R_X2].empty() || !tp_c2r_2d.empty();
+        const bool side = !profiling && any_c2r;
         const Space<T> *dsp = d_spaces.as<Space<T>>();
         if (side) {
             fork(ev_fork);
@@ -898,7 +929,8 @@ class Engine : public EngineBase {
         } else {
             timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
         }
-        if (tp[PASS_C2R_X2].empty()) return;
+        if (!any_c2r) return;
+        run_transform(PASS_C2R_2D, tp_c2r_2d, sel, KK_C2R_2D);
         run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
         run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
         const int kks[5] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_SGS_MLP, KK_SGS_STRESS};

@@ -19,6 +19,7 @@
 
 #include "fft.hpp"
 #include "terms.hpp"
+#include "transforms_2d.hpp"
 #include "transforms_reg.hpp"
 
 namespace bld {
@@ -108,6 +109,20 @@ __global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spa
         c2r_x2_reg<T, n>(S, E, sel, smem);                 \
     else                                              \
         c2r_x2_body<T, n>(S, E, sel, smem)
+    BLD_FFT_SWITCH(S.N2, BLD_CALL)
+#undef BLD_CALL
+}
+
+// fused c2r of whole (term, layer) planes of square grids, one plane per block
+template <typename T>
+__global__ void __launch_bounds__(kC2r2dThreads) k_c2r_2d(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n)                                 \
+    if constexpr (c2r2d_supported<T>(n))           \
+        c2r_2d_reg<T, n>(S, E, sel, smem)
     BLD_FFT_SWITCH(S.N2, BLD_CALL)
 #undef BLD_CALL
 }
@@ -1062,7 +1077,15 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
         case PASS_C2R_X1: k_c2r_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_R2C_X1: k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
+        case PASS_C2R_2D: k_c2r_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt, sel); break;
     }
+}
+
+// the fused c2r keeps a whole plane in LDS: allow up to 160 KiB per block
+template <typename T>
+hipError_t allow_large_lds() {
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 
 template <typename T>
@@ -1128,7 +1151,8 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
     template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t); \
     template void launch_stage_sequential<T>(const ProjDesc<T> &, int, double, hipStream_t);                      \
     template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \
-    template void launch_fill<T>(void *, size_t, hipStream_t);
+    template void launch_fill<T>(void *, size_t, hipStream_t);                                                    \
+    template hipError_t allow_large_lds<T>();
 
 BLD_INSTANTIATE(float)
 BLD_INSTANTIATE(double)

@@ -1,0 +1,155 @@
+// Fused two-dimensional inverse transform of one (term, layer) plane per block
+// (physical_space.jl:105-145 for both directions at once): the x2 column
+// FFTs write the intermediate Z[n2][k1] into LDS, the x1 row FFTs read it
+// there and write the physical plane.  The intermediate never goes to HBM,
+// which removes its write and re-read from the two-pass path.  Square grids
+// (N1 = N2 = N) with a register FFT for N; the plane's intermediate has to fit
+// in LDS next to the exchange buffer of one round of row pairs.
+#pragma once
+#include "transforms_reg.hpp"
+
+namespace bld {
+
+// LDS layout of the fused c2r: Z[N][zs] (zs = nk1 rounded up to an odd
+// multiple of 2 for bank spread), then the row-pair exchange X[P][B·(A+1)]
+__host__ __device__ constexpr int c2r2d_zs(int nk1) { return (nk1 % 4 == 0) ? nk1 + 2 : nk1 + (nk1 & 1); }
+__host__ __device__ constexpr int c2r2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
+__host__ __device__ constexpr size_t c2r2d_lds(int N, int nk1, size_t elem) {
+    return (size_t(N) * c2r2d_zs(nk1) +
+            size_t(c2r2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0)) *
+           elem;
+}
+
+// lengths with an instantiated fused c2r (larger planes do not fit in LDS)
+template <typename T>
+__host__ __device__ constexpr bool c2r2d_supported(int N) {
+    return reg_fft_a(N) > 0 && N % 2 == 0 && N <= (sizeof(T) == 4 ? 192 : 96);
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
+    constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1, PS = Bq * RS;
+    constexpr int P = c2r2d_pairs(N), NP = N / 2;
+    const int job = blockIdx.x - E.block_begin;
+    int term = 0;
+    while (job >= S.in_job0[term + 1]) ++term;
+    const int layer = job - S.in_job0[term];
+    const int recipe = S.in_recipe[term];
+    const int nk1 = S.nk1, zs = c2r2d_zs(nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    cplx<T> *Z = smem;
+    cplx<T> *X = smem + size_t(N) * zs;
+    const size_t P2 = size_t(K1) * K2;
+    const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
+
+    // x2, stage 1: item (c, t), c fastest; A values at k2 positions Bq·n + t
+    {
+        const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;
+        for (int it = threadIdx.x; it < nk1 * Bq; it += blockDim.x) {
+            const int t = it / nk1, c = it - t * nk1;
+            int i2s[A];
+            bool ok[A];
+            static_for<0, A>([&](auto n1c) {
+                constexpr int n1 = decltype(n1c)::value;
+                const int pos = Bq * n1 + t;
+                const bool lo = pos <= k2m, hi = pos >= N - k2m;
+                ok[n1] = lo || hi;
+                i2s[n1] = lo ? pos : (hi ? K2 - (N - pos) : 0);
+            });
+            const size_t base = size_t(layer) * P2 + c;
+            cplx<T> v[A];
+            auto gather = [&](auto value) {
+                static_for<0, A>([&](auto n1c) {
+                    constexpr int n1 = decltype(n1c)::value;
+                    const cplx<T> x = value(base + size_t(K1) * i2s[n1], i2s[n1]);
+                    v[n1] = ok[n1] ? x : czero<T>();
+                });
+            };
+            const T d1 = T(gload(S.D1 + c));
+            switch (recipe) {  // uniform per block
+                case R_U: gather([&](size_t i, int) { return gload(u + i); }); break;
+                case R_V: gather([&](size_t i, int) { return gload(vv + i); }); break;
+                case R_W: gather([&](size_t i, int) { return gload(w + i); }); break;
+                case R_D1U: gather([&](size_t i, int) { return imul(d1, gload(u + i)); }); break;
+                case R_D1V: gather([&](size_t i, int) { return imul(d1, gload(vv + i)); }); break;
+                case R_D1W: gather([&](size_t i, int) { return imul(d1, gload(w + i)); }); break;
+                case R_D2U: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(u + i)); }); break;
+                case R_D2V: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(vv + i)); }); break;
+                case R_D2W: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(w + i)); }); break;
+                default: gather([&](size_t i, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
+            }
+            rdft<T, true, A>(v);
+            static_for<0, A>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                Z[(t * A + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw, t * k));
+            });
+        }
+    }
+    __syncthreads();
+    // x2, stage 2: item (c, t) reads rows n·A + t and writes rows t + A·k, the
+    // same set, so the update is in place
+    for (int it = threadIdx.x; it < nk1 * A; it += blockDim.x) {
+        const int t = it / nk1, c = it - t * nk1;
+        cplx<T> v[Bq];
+        static_for<0, Bq>([&](auto nc) {
+            constexpr int n = decltype(nc)::value;
+            v[n] = Z[(n * A + t) * zs + c];
+        });
+        rdft<T, true, Bq>(v);
+        static_for<0, Bq>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            cplx<T> val = v[k];
+            if (c == 0) val.y = T(0);  // c2r uses the real part of the k1 = 0 column
+            Z[(t + A * k) * zs + c] = val;
+        });
+    }
+    __syncthreads();
+    // x1: rounds of P row pairs, Hermitian completion of (row 2p) + i·(row 2p+1)
+    const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
+    T *out = S.in_phys[term] + size_t(layer) * N * N;
+    for (int p0 = 0; p0 < NP; p0 += P) {
+        for (int it = threadIdx.x; it < P * Bq; it += blockDim.x) {
+            const int pl = it / Bq, t = it - pl * Bq;
+            const int p = min(p0 + pl, NP - 1);
+            const cplx<T> *ra = Z + size_t(2 * p) * zs, *rb = ra + zs;
+            cplx<T> v[A];
+            static_for<0, A>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                const int k = Bq * n + t;
+                const bool direct = k <= k1m, mirror = !direct && N - k <= k1m;
+                const bool real_only = k == 0 || (2 * k == N && direct);
+                const int kk = direct ? k : (mirror ? N - k : 0);
+                const cplx<T> a = ra[kk], b = rb[kk];
+                cplx<T> z = real_only ? mk(a.x, b.x) : direct ? mk(a.x - b.y, a.y + b.x) : mk(a.x + b.y, b.x - a.y);
+                if (!(direct || mirror)) z = czero<T>();
+                v[n] = z;
+            });
+            rdft<T, true, A>(v);
+            static_for<0, A>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw1, t * k));
+            });
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < P * A; it += blockDim.x) {
+            const int pl = it / A, t = it - pl * A;
+            const int p = p0 + pl;
+            cplx<T> v[Bq];
+            static_for<0, Bq>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                v[n] = X[pl * PS + n * RS + t];
+            });
+            rdft<T, true, Bq>(v);
+            if (p < NP) {
+                T *ra = out + size_t(2 * p) * N, *rb = ra + N;
+                static_for<0, Bq>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    gstore(ra + t + A * k, v[k].x);
+                    gstore(rb + t + A * k, v[k].y);
+                });
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace bld

@@ -94,6 +94,24 @@ def spectral_noise(model, seed=0, amp=0.25, k0=3.0):
     return out
 
 
+REG_FFT_A = {16: 4, 24: 6, 32: 8, 48: 8, 64: 8, 72: 9, 96: 12, 128: 16, 144: 12, 192: 16, 256: 16, 384: 24}
+
+
+def c2r_fused(N1, N2, nk1, precision):
+    """Whether the engine transforms a space's planes with the fused c2r
+    kernel: square grid, even length with a register FFT, at most 192 (fp32)
+    or 96 (fp64) points, and the plane's intermediate plus one round of row
+    pairs within 160 KiB of LDS (the engine's own rule)."""
+    N = N2
+    if N1 != N2 or N % 2 or N not in REG_FFT_A or N > (192 if precision == "fp32" else 96):
+        return False
+    A = REG_FFT_A[N]
+    zs = nk1 + 2 if nk1 % 4 == 0 else nk1 + (nk1 & 1)
+    pairs = 32 if N > 128 else min(N // 2, 64)
+    elem = 8 if precision == "fp32" else 16
+    return (N * zs + pairs * (N // A) * (A + 1)) * elem <= 160 * 1024
+
+
 def algorithmic_bytes(cfg_sizes, kind, precision):
     """Minimal HBM bytes of each kernel class per right-hand side over all
     spaces (stage_project: per stage; inputs read once, outputs written once).
@@ -112,10 +130,13 @@ def algorithmic_bytes(cfg_sizes, kind, precision):
 
     terms = {"adv": (3 * nC + 3 * nI), "sgs": (6 * nC + 3 * nI)}
     srcs = {"adv": (2 * nC + nI), "sgs": (5 * nC + 3 * nI)}
+    fused = [sp for sp in spaces if c2r_fused(sp[1], sp[2], nk1(sp[1]), precision)]
+    paired = [sp for sp in spaces if sp not in fused]
     out = {}
     out["linear"] = 2 * S
-    out["c2r_x2"] = sum(terms[k] * (nk * bc + N2 * nk1(N1) * bc) for k, N1, N2 in spaces)
-    out["c2r_x1"] = sum(terms[k] * (N2 * nk1(N1) * bc + N1 * N2 * br) for k, N1, N2 in spaces)
+    out["c2r_2d"] = sum(terms[k] * (nk * bc + N1 * N2 * br) for k, N1, N2 in fused)
+    out["c2r_x2"] = sum(terms[k] * (nk * bc + N2 * nk1(N1) * bc) for k, N1, N2 in paired)
+    out["c2r_x1"] = sum(terms[k] * (N2 * nk1(N1) * bc + N1 * N2 * br) for k, N1, N2 in paired)
     out["r2c_x1"] = sum(srcs[k] * (N1 * N2 * br + N2 * nk1(N1) * bc) for k, N1, N2 in spaces)
     r2c_x2 = []
     for k, N1, N2 in spaces:

@@ -54,6 +54,12 @@ CASES = {
     "les": dict(res=(16, 16, 8), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4), upper="FreeSlipBoundary",
                 procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
                                                       sgs_model=M.StaticSmagorinskyModel(Cs=0.1, wall_damping=True))),
+    # square LES grids (advection 48x48, SGS 32x32): the fused whole-plane c2r
+    # with more than one round of row pairs
+    "les_square": dict(res=(32, 32, 8), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4),
+                       upper="FreeSlipBoundary",
+                       procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
+                                                             sgs_model=M.StaticSmagorinskyModel(Cs=0.1))),
     # closed channel LES with two rough walls, no damping (les_test.jl:1-48)
     "les_closed": dict(res=(12, 14, 6), dims=(2 * np.pi, 4 / 3 * np.pi, 2.0), lower=("RoughWall", 5e-3),
                        upper=("RoughWall", 5e-3),
@@ -120,7 +126,7 @@ def test_rates_fp64(name):
         assert relerr(got[f], rates[f]) < RTOL_RHS64, (name, f, relerr(got[f], rates[f]))
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "les_square"])
 def test_rates_nonhermitian_input(name):
     """c2r semantics for a k1=0 column that is not Hermitian (FFTW/pocketfft
     take the real part, physical_space.jl:293)."""
@@ -172,7 +178,7 @@ def test_physical_transfer(name):
         assert relerr(gpu.state[f], exp) < 1e-13
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "les_square"])
 def test_rates_fp32_mixed(name):
     ref, gpu = pair(name, precision="fp32", seed=9)
     rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
