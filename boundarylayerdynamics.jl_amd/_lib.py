@@ -24,7 +24,7 @@ I_PROFILES = ("vel3", "adv3", "sgs13", "sgs23")
 
 # kernel classes reported by bld_kernel_times (order of the engine's KernelKind)
 KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "sgs_stress", "r2c_x1",
-                "r2c_x2", "stage_project", "check_finite", "sgs_mlp", "c2r_2d")
+                "r2c_x2", "stage_project", "check_finite", "sgs_mlp", "c2r_2d", "r2c_2d")
 
 # every symbol include/bld.h declares
 EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_size", "bld_select_instance",

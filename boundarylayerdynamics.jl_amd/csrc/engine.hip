@@ -161,7 +161,7 @@ struct EngineBase {
 
 enum KernelKind {
     KK_LINEAR = 0, KK_C2R_X2, KK_C2R_X1, KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_R2C_X1, KK_R2C_X2,
-    KK_STAGE, KK_CHECK, KK_SGS_MLP, KK_C2R_2D, KK_COUNT
+    KK_STAGE, KK_CHECK, KK_SGS_MLP, KK_C2R_2D, KK_R2C_2D, KK_COUNT
 };
 
 // One model (one bld_config): its device state, constants and descriptors.
@@ -296,7 +296,10 @@ class Engine : public EngineBase {
     std::vector<Pass> tp[4];
     std::vector<Pass> tp_r2c_x2_sgs;
     std::vector<Pass> tp_c2r_2d;  // fused c2r of whole planes (square grids that fit in LDS)
-    bool fused_c2r = true;
+    std::vector<Pass> tp_r2c_2d, tp_r2c_2d_sgs;  // fused r2c (advection, SGS: both accumulate into du)
+    bool fused_c2r = true, fused_r2c = true;
+    int tuned_mode = 3;
+    double tuned_us_rhs = 0.0;
     std::vector<std::pair<int, int>> tp_entry_space[5];  // (space, batch) per pass, for bld_tuning
     Pass pp[5];  // physics: advection products, SGS viscosity, SGS stress, learned stress, its rates
     Pass lin_pass, stage_pass, stage00_pass, xfer[4];
@@ -679,14 +682,21 @@ class Engine : public EngineBase {
         return c2r2d_lds(N, S.nk1, sizeof(cplx<T>)) <= 160 * 1024;
     }
 
+    // the fused r2c in addition keeps one x2 item of each stage per thread
+    static bool r2c_2d_fits(const Space<T> &S) {
+        if (!c2r_2d_fits(S)) return false;
+        const int A = reg_fft_a(S.N2), Bq = S.N2 / A;
+        return S.nk1 * std::max(A, Bq) <= kC2r2dThreads;
+    }
+
     void add_entry(Pass &ps, int kind, int sidx, int batch = 0) {
         const Space<T> &S = spaces[sidx];
-        if (kind == PASS_C2R_2D) {  // one block per (term, layer) plane
+        if (kind == PASS_C2R_2D || kind == PASS_R2C_2D) {  // one block per plane
             LaunchEntry E{};
             E.space = sidx;
             E.batch = 1;
             E.blocks_per_job = 1;
-            E.njobs = S.in_job0[S.nin];
+            E.njobs = kind == PASS_C2R_2D ? S.in_job0[S.nin] : S.tgt_job0[S.ntgt];
             E.block_begin = ps.nblocks;
             ps.ent.push_back(E);
             ps.kind.push_back(kind);
@@ -751,6 +761,8 @@ class Engine : public EngineBase {
         for (int k = 0; k < 4; ++k) tp[k].clear();
         tp_r2c_x2_sgs.clear();
         tp_c2r_2d.clear();
+        tp_r2c_2d.clear();
+        tp_r2c_2d_sgs.clear();
         for (int k = 0; k < 5; ++k) tp_entry_space[k].clear();
         std::vector<int> model_spaces;
         for (auto &I : inst)
@@ -759,16 +771,18 @@ class Engine : public EngineBase {
         for (size_t i = 0; i < model_spaces.size(); ++i) {
             const int s = model_spaces[i];
             auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
+            const bool adv = spaces[s].kind == SK_ADVECTION;
             if (fused_c2r && c2r_2d_fits(spaces[s])) {
                 bucket_add(tp_c2r_2d, PASS_C2R_2D, s, 0);
-                bucket_add(tp[PASS_R2C_X1], PASS_R2C_X1, s, b(PASS_R2C_X1));
             } else {
-                for (int k = 0; k < 3; ++k) bucket_add(tp[k], k, s, b(k));
+                for (int k = 0; k < 2; ++k) bucket_add(tp[k], k, s, b(k));
             }
-            if (spaces[s].kind == SK_ADVECTION)
-                bucket_add(tp[PASS_R2C_X2], PASS_R2C_X2, s, b(3));
-            else
-                bucket_add(tp_r2c_x2_sgs, PASS_R2C_X2, s, b(3));
+            if (fused_r2c && r2c_2d_fits(spaces[s])) {
+                bucket_add(adv ? tp_r2c_2d : tp_r2c_2d_sgs, PASS_R2C_2D, s, 0);
+            } else {
+                bucket_add(tp[PASS_R2C_X1], PASS_R2C_X1, s, b(PASS_R2C_X1));
+                bucket_add(adv ? tp[PASS_R2C_X2] : tp_r2c_x2_sgs, PASS_R2C_X2, s, b(3));
+            }
             for (int k = 0; k < 4; ++k) {
                 int n, bufs, avail, njobs;
                 pass_geometry(k, spaces[s], n, bufs, avail, njobs);
@@ -778,7 +792,9 @@ class Engine : public EngineBase {
         for (int k = 0; k < 4; ++k) bucket_finish(tp[k]);
         bucket_finish(tp_r2c_x2_sgs);
         bucket_finish(tp_c2r_2d);
-        if (!tp_c2r_2d.empty()) HIPCHECK(allow_large_lds<T>());
+        bucket_finish(tp_r2c_2d);
+        bucket_finish(tp_r2c_2d_sgs);
+        if (!tp_c2r_2d.empty() || !tp_r2c_2d.empty() || !tp_r2c_2d_sgs.empty()) HIPCHECK(allow_large_lds<T>());
     }
 
     void build_plans() {
@@ -947,7 +963,9 @@ class Engine : public EngineBase {
         if (side) join(ev_join2);
         run_transform(PASS_R2C_X1, tp[PASS_R2C_X1], sel, KK_R2C_X1);
         run_transform(PASS_R2C_X2, tp[PASS_R2C_X2], sel, KK_R2C_X2);
+        run_transform(PASS_R2C_2D, tp_r2c_2d, sel, KK_R2C_2D);
         run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
+        run_transform(PASS_R2C_2D, tp_r2c_2d_sgs, sel, KK_R2C_2D);
     }
 
     // The float64 (0,0) pass of the mixed form touches only the (0,0) columns,
@@ -1269,6 +1287,36 @@ class Engine : public EngineBase {
         return 1e3 * ms / reps;
     }
 
+    // time of one call of f (µs): the fastest of three timed groups of reps
+    // calls after a warm-up call, so that one disturbed group does not decide
+    template <typename F>
+    double time_us(F f, int reps) {
+        f();
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        double best = 1e30;
+        for (int g = 0; g < 3; ++g) {
+            HIPCHECK(hipEventRecord(a, st));
+            for (int r = 0; r < reps; ++r) f();
+            HIPCHECK(hipEventRecord(b, st));
+            HIPCHECK(hipEventSynchronize(b));
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, a, b));
+            best = std::min(best, 1e3 * ms / reps);
+        }
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        return best;
+    }
+
+    bool any_fused_space() const {
+        for (auto &I : inst)
+            for (int s : {I->adv, I->sgs})
+                if (s >= 0 && c2r_2d_fits(spaces[s])) return true;
+        return false;
+    }
+
     void autotune(int reps) override {
         if (reps < 1) reps = 5;
         std::vector<int> model_spaces;
@@ -1296,7 +1344,29 @@ class Engine : public EngineBase {
                 tuned_us[k][i] = bt;
             }
         }
-        build_transform_passes(&best);
+        // fused or two-pass transforms in each direction (bit 0: fused c2r,
+        // bit 1: fused r2c): whole right-hand sides on the live state with each
+        // choice, the fastest kept
+        {
+            double bt = 1e30;
+            int bm = 3;
+            for (int mode = 3; mode >= 0; --mode) {
+                fused_c2r = (mode & 1) != 0;
+                fused_r2c = (mode & 2) != 0;
+                build_transform_passes(&best);
+                if (mode < 3 && !any_fused_space()) break;  // nothing qualifies: one choice only
+                const double us = time_us([&] { rhs(0); }, reps);
+                if (us < bt) {
+                    bt = us;
+                    bm = mode;
+                }
+            }
+            fused_c2r = (bm & 1) != 0;
+            fused_r2c = (bm & 2) != 0;
+            tuned_mode = bm;
+            tuned_us_rhs = bt;
+            build_transform_passes(&best);
+        }
         // chunk length of the stage kernel: stage 1 only writes utmp, so it can be
         // replayed on the live state
         int n3max = 0;
@@ -1307,20 +1377,9 @@ class Engine : public EngineBase {
             if (n3max > 64 * cl) continue;
             stage_CL_tuned = cl;
             build_point_plans();
-            stage(1, 1e-6);
-            hipEvent_t a, b;
-            HIPCHECK(hipEventCreate(&a));
-            HIPCHECK(hipEventCreate(&b));
-            HIPCHECK(hipEventRecord(a, st));
-            for (int r = 0; r < reps; ++r) stage(1, 1e-6);
-            HIPCHECK(hipEventRecord(b, st));
-            HIPCHECK(hipEventSynchronize(b));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, a, b));
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            if (1e3 * ms / reps < best_us) {
-                best_us = 1e3 * ms / reps;
+            const double us = time_us([&] { stage(1, 1e-6); }, reps);
+            if (us < best_us) {
+                best_us = us;
                 best_cl = cl;
             }
         }
@@ -1339,6 +1398,11 @@ class Engine : public EngineBase {
         if (pass == 4) {  // stage kernel: chunk length and one stage's time
             *batch = stage_CL;
             *us = tuned_us[4].empty() ? 0.0 : tuned_us[4][0];
+            return;
+        }
+        if (pass == 5) {  // transform path: bit 0 fused c2r, bit 1 fused r2c; one right-hand side's time
+            *batch = (fused_c2r ? 1 : 0) | (fused_r2c ? 2 : 0);
+            *us = tuned_us_rhs;
             return;
         }
         if (pass < 0 || pass > 3) throw BldError{BLD_EARG, "unknown pass"};

@@ -127,6 +127,20 @@ __global__ void __launch_bounds__(kC2r2dThreads) k_c2r_2d(const Space<T> *__rest
 #undef BLD_CALL
 }
 
+// fused r2c of whole (target, layer) planes of square grids, one plane per block
+template <typename T>
+__global__ void __launch_bounds__(kC2r2dThreads) k_r2c_2d(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n)                       \
+    if constexpr (c2r2d_supported<T>(n)) \
+        r2c_2d_reg<T, n>(S, E, smem)
+    BLD_FFT_SWITCH(S.N2, BLD_CALL)
+#undef BLD_CALL
+}
+
 // ---------------------------------------------------------------------------
 // c2r, pass 2: Hermitian completion along x1, two real rows per complex FFT
 // ---------------------------------------------------------------------------
@@ -1078,13 +1092,17 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
         case PASS_R2C_X1: k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_C2R_2D: k_c2r_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt, sel); break;
+        case PASS_R2C_2D: k_r2c_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt); break;
     }
 }
 
-// the fused c2r keeps a whole plane in LDS: allow up to 160 KiB per block
+// the fused transforms keep a whole plane in LDS: allow up to 160 KiB per block
 template <typename T>
 hipError_t allow_large_lds() {
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_r2c_2d<T>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
 

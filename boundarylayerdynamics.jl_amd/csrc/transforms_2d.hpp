@@ -1,26 +1,34 @@
-// Fused two-dimensional inverse transform of one (term, layer) plane per block
-// (physical_space.jl:105-145 for both directions at once): the x2 column
-// FFTs write the intermediate Z[n2][k1] into LDS, the x1 row FFTs read it
-// there and write the physical plane.  The intermediate never goes to HBM,
-// which removes its write and re-read from the two-pass path.  Square grids
-// (N1 = N2 = N) with a register FFT for N; the plane's intermediate has to fit
-// in LDS next to the exchange buffer of one round of row pairs.
+// Fused two-dimensional transforms of whole planes, one plane per block, for
+// square grids (N1 = N2 = N) with a register FFT for N.  The c2r (one (term,
+// layer) plane, physical_space.jl:277-295) runs the x2 column FFTs into an
+// LDS-resident intermediate Z[n2][k1] and the x1 row FFTs from there to the
+// physical plane; the r2c (one (target, layer) rate plane with all its
+// sources, physical_space.jl:208-246) runs the same passes in the opposite
+// order.  The intermediate never goes to HBM, which removes its write and
+// re-read from the two-pass path; it has to fit in LDS next to the exchange
+// buffer of one round of row pairs.
 #pragma once
 #include "transforms_reg.hpp"
 
 namespace bld {
 
-// LDS layout of the fused c2r: Z[N][zs] (zs = nk1 rounded up to an odd
-// multiple of 2 for bank spread), then the row-pair exchange X[P][B·(A+1)]
-__host__ __device__ constexpr int c2r2d_zs(int nk1) { return (nk1 % 4 == 0) ? nk1 + 2 : nk1 + (nk1 & 1); }
+// LDS layout of the fused transforms: the intermediate Z[N][zs], then the
+// row-pair exchange X[P][B·(A+1)].  The row items of a pair are B consecutive
+// lanes reading B consecutive entries of two Z rows; with zs ≡ B/2 (mod 16)
+// the pairs of a half wave start 2·B banks apart, so their reads of 8-byte
+// entries fall on disjoint banks.
+__host__ __device__ constexpr int c2r2d_zs(int N, int nk1) {
+    const int bh = reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) / 2 : 0;
+    return nk1 + (((bh - nk1) % 16) + 16) % 16;
+}
 __host__ __device__ constexpr int c2r2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
 __host__ __device__ constexpr size_t c2r2d_lds(int N, int nk1, size_t elem) {
-    return (size_t(N) * c2r2d_zs(nk1) +
+    return (size_t(N) * c2r2d_zs(N, nk1) +
             size_t(c2r2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0)) *
            elem;
 }
 
-// lengths with an instantiated fused c2r (larger planes do not fit in LDS)
+// lengths with instantiated fused transforms (larger planes do not fit in LDS)
 template <typename T>
 __host__ __device__ constexpr bool c2r2d_supported(int N) {
     return reg_fft_a(N) > 0 && N % 2 == 0 && N <= (sizeof(T) == 4 ? 192 : 96);
@@ -35,7 +43,7 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     while (job >= S.in_job0[term + 1]) ++term;
     const int layer = job - S.in_job0[term];
     const int recipe = S.in_recipe[term];
-    const int nk1 = S.nk1, zs = c2r2d_zs(nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;
     const size_t P2 = size_t(K1) * K2;
@@ -150,6 +158,140 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
         }
         __syncthreads();
     }
+}
+
+// Fused forward transform of every source of one (target, layer) rate plane
+// per block.  Per source the x1 row FFTs (two real rows per complex FFT)
+// produce the half spectra of the rows, which are added into Z[n2][k1] in
+// LDS, with the i·D1 prefactor where the target asks for it.  The x2 FFTs then
+// run in place on Z, and the resolved modes are scaled by 1/(N1·N2) and added
+// to the target.  Sources that take i·D2 after the transform form a second
+// sum, transformed the same way and added afterwards.  Neither the row spectra
+// nor the sums go to HBM.
+template <typename T, int N>
+__device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
+    constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1, PS = Bq * RS;
+    constexpr int P = c2r2d_pairs(N), NP = N / 2;
+    const int job = blockIdx.x - E.block_begin;
+    int tg = 0;
+    while (job >= S.tgt_job0[tg + 1]) ++tg;
+    const int layer = job - S.tgt_job0[tg];
+    const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
+    cplx<T> *Z = smem;
+    cplx<T> *X = smem + size_t(N) * zs;
+    const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
+    const cplx<T> *__restrict__ tw2 = S.tw + S.p2.tw;
+    const int tid = threadIdx.x;
+    const T scale = T(1.0 / (double(N) * double(N)));
+    cplx<T> *out = S.tgt_out[tg] + size_t(layer) * K1 * K2;
+
+    // x1 transform of source `src` added into Z (first = overwrite)
+    auto rows = [&](int src, bool dx, bool first) {
+        const T *in = S.src_phys[src] + size_t(layer) * N * N;
+        for (int p0 = 0; p0 < NP; p0 += P) {
+            // stage 1: item (pair, t), A values of both rows at x = Bq·n + t
+            for (int it = tid; it < P * Bq; it += blockDim.x) {
+                const int pl = it / Bq, t = it - pl * Bq;
+                const int p = min(p0 + pl, NP - 1);
+                const T *ra = in + size_t(2 * p) * N, *rb = ra + N;
+                cplx<T> v[A];
+                static_for<0, A>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    v[n] = mk(gload(ra + Bq * n + t), gload(rb + Bq * n + t));
+                });
+                rdft<T, false, A>(v);
+                static_for<0, A>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, false>(tw1, t * k));
+                });
+            }
+            __syncthreads();
+            // stage 2 in place: spectrum index t + A·k of the pair at X[pl][k·RS + t]
+            for (int it = tid; it < P * A; it += blockDim.x) {
+                const int pl = it / A, t = it - pl * A;
+                cplx<T> v[Bq];
+                static_for<0, Bq>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    v[n] = X[pl * PS + n * RS + t];
+                });
+                rdft<T, false, Bq>(v);
+                static_for<0, Bq>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    X[pl * PS + k * RS + t] = v[k];
+                });
+            }
+            __syncthreads();
+            // split the pair spectrum z = a + i·b into the half spectra of the rows
+            for (int it = tid; it < P * nk1; it += blockDim.x) {
+                const int pl = it / nk1, k = it - pl * nk1;
+                const int p = p0 + pl;
+                const int kn = k == 0 ? 0 : N - k;
+                const cplx<T> zk = X[pl * PS + (k / A) * RS + k % A];
+                const cplx<T> zn = X[pl * PS + (kn / A) * RS + kn % A];
+                if (p < NP) {
+                    cplx<T> ra = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
+                    cplx<T> rb = mk(T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x));
+                    if (dx) {
+                        const T d1 = T(gload(S.D1 + k));
+                        ra = imul(d1, ra);
+                        rb = imul(d1, rb);
+                    }
+                    cplx<T> *za = Z + size_t(2 * p) * zs + k, *zb = za + zs;
+                    *za = first ? ra : *za + ra;
+                    *zb = first ? rb : *zb + rb;
+                }
+            }
+            __syncthreads();
+        }
+    };
+    // x2 transform of Z in place; the resolved modes (times i·D2 with d2) are
+    // scaled and added to the target
+    auto columns = [&](bool d2) {
+        const bool own1 = tid < nk1 * Bq;
+        const int t1 = tid / nk1, c1 = tid - t1 * nk1;
+        cplx<T> v[A];
+        if (own1) {
+            static_for<0, A>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                v[n] = Z[(Bq * n + t1) * zs + c1];
+            });
+        }
+        __syncthreads();
+        if (own1) {
+            rdft<T, false, A>(v);
+            static_for<0, A>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                Z[(t1 * A + k) * zs + c1] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, false>(tw2, t1 * k));
+            });
+        }
+        __syncthreads();
+        const int t2 = tid / nk1, c2 = tid - t2 * nk1;
+        if (tid < nk1 * A) {
+            cplx<T> w[Bq];
+            static_for<0, Bq>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                w[n] = Z[(n * A + t2) * zs + c2];
+            });
+            rdft<T, false, Bq>(w);
+            static_for<0, Bq>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                const int pos = t2 + A * k;
+                const bool lo = pos <= k2m, hi = pos >= N - k2m;
+                if (lo || hi) {
+                    const int i2 = lo ? pos : K2 - (N - pos);
+                    const cplx<T> val = d2 ? imul(T(gload(S.D2 + i2)), w[k]) : w[k];
+                    const size_t o = size_t(i2) * K1 + c2;
+                    gstore(out + o, gload(out + o) + scale * val);
+                }
+            });
+        }
+        __syncthreads();
+    };
+    for (int s = 0; s < nG; ++s) rows(S.tgt_G[tg][s], S.tgt_Gd1[tg][s] != 0, s == 0);
+    if (nG > 0) columns(false);
+    for (int s = 0; s < nH; ++s) rows(S.tgt_H[tg][s], false, s == 0);
+    if (nH > 0) columns(true);
 }
 
 }  // namespace bld

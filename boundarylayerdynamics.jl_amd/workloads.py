@@ -106,10 +106,18 @@ def c2r_fused(N1, N2, nk1, precision):
     if N1 != N2 or N % 2 or N not in REG_FFT_A or N > (192 if precision == "fp32" else 96):
         return False
     A = REG_FFT_A[N]
-    zs = nk1 + 2 if nk1 % 4 == 0 else nk1 + (nk1 & 1)
+    zs = nk1 + ((N // A) // 2 - nk1) % 16
     pairs = 32 if N > 128 else min(N // 2, 64)
     elem = 8 if precision == "fp32" else 16
     return (N * zs + pairs * (N // A) * (A + 1)) * elem <= 160 * 1024
+
+
+def r2c_fused(N1, N2, nk1, precision):
+    """The fused r2c in addition keeps one x2 item per thread (1024 threads)."""
+    if not c2r_fused(N1, N2, nk1, precision):
+        return False
+    A = REG_FFT_A[N2]
+    return nk1 * max(A, N2 // A) <= 1024
 
 
 def algorithmic_bytes(cfg_sizes, kind, precision):
@@ -137,9 +145,18 @@ def algorithmic_bytes(cfg_sizes, kind, precision):
     out["c2r_2d"] = sum(terms[k] * (nk * bc + N1 * N2 * br) for k, N1, N2 in fused)
     out["c2r_x2"] = sum(terms[k] * (nk * bc + N2 * nk1(N1) * bc) for k, N1, N2 in paired)
     out["c2r_x1"] = sum(terms[k] * (N2 * nk1(N1) * bc + N1 * N2 * br) for k, N1, N2 in paired)
-    out["r2c_x1"] = sum(srcs[k] * (N1 * N2 * br + N2 * nk1(N1) * bc) for k, N1, N2 in spaces)
+    fused_r = [sp for sp in spaces if r2c_fused(sp[1], sp[2], nk1(sp[1]), precision)]
+    paired_r = [sp for sp in spaces if sp not in fused_r]
+    out["r2c_x1"] = sum(srcs[k] * (N1 * N2 * br + N2 * nk1(N1) * bc) for k, N1, N2 in paired_r)
+    # fused r2c: every source plane of a target read (T12 serves two targets),
+    # the target read and written
+    r2c_2d = 0
+    for k, N1, N2 in fused_r:
+        per_tgt = [(1, nC), (1, nC), (1, nI)] if k == "adv" else [(3, nC), (3, nC), (3, nI)]
+        r2c_2d += sum(ns * L * N1 * N2 * br + 2 * L * nk * bc for ns, L in per_tgt)
+    out["r2c_2d"] = r2c_2d
     r2c_x2 = []
-    for k, N1, N2 in spaces:
+    for k, N1, N2 in paired_r:
         per_tgt = [(1, nC), (1, nC), (1, nI)] if k == "adv" else [(3, nC), (3, nC), (3, nI)]
         r2c_x2.append(sum(ns * L * N2 * nk1(N1) * bc + 2 * L * nk * bc for ns, L in per_tgt))
     out["r2c_x2"] = sum(r2c_x2)

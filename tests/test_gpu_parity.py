@@ -372,7 +372,7 @@ def test_ensemble_fp32_matches_single_models():
             assert relerr(a[f], b[f]) < 1e-5, (f, relerr(a[f], b[f]))
 
 
-@pytest.mark.parametrize("name", ["les", "channel"])
+@pytest.mark.parametrize("name", ["les", "channel", "les_square"])
 def test_autotune_keeps_results(name):
     ref, gpu = pair(name, seed=60)
     before = gpu.rates()
