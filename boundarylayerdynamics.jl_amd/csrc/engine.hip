@@ -950,9 +950,16 @@ class Engine : public EngineBase {
         run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
         run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
         const int kks[5] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_SGS_MLP, KK_SGS_STRESS};
+        bool adv_r2c_done = false;
         if (side && pp[PHYS_ADV].nblocks) {
             fork(ev_fork2);
             launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st2);
+            // whole-plane r2c of the advection rates: accumulates into du behind
+            // the linear terms on the side stream while the SGS kernels run; the
+            // SGS r2c accumulates after the join
+            for (const Pass &p : tp_r2c_2d)
+                launch_transform<T>(PASS_R2C_2D, dsp, p.lt(), p.nblocks, p.smem, sel, st2);
+            adv_r2c_done = true;
         } else if (pp[PHYS_ADV].nblocks) {
             timed(KK_ADV, [&] { launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st); });
         }
@@ -963,7 +970,7 @@ class Engine : public EngineBase {
         if (side) join(ev_join2);
         run_transform(PASS_R2C_X1, tp[PASS_R2C_X1], sel, KK_R2C_X1);
         run_transform(PASS_R2C_X2, tp[PASS_R2C_X2], sel, KK_R2C_X2);
-        run_transform(PASS_R2C_2D, tp_r2c_2d, sel, KK_R2C_2D);
+        if (!adv_r2c_done) run_transform(PASS_R2C_2D, tp_r2c_2d, sel, KK_R2C_2D);
         run_transform(PASS_R2C_X2, tp_r2c_x2_sgs, sel, KK_R2C_X2);
         run_transform(PASS_R2C_2D, tp_r2c_2d_sgs, sel, KK_R2C_2D);
     }

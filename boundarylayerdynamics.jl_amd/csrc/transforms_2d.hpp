@@ -22,10 +22,19 @@ __host__ __device__ constexpr int c2r2d_zs(int N, int nk1) {
     return nk1 + (((bh - nk1) % 16) + 16) % 16;
 }
 __host__ __device__ constexpr int c2r2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
+// ... followed by the N twiddles w_N^j (one table serves both directions of
+// a square grid), read in the stages whose data come from LDS
 __host__ __device__ constexpr size_t c2r2d_lds(int N, int nk1, size_t elem) {
     return (size_t(N) * c2r2d_zs(N, nk1) +
-            size_t(c2r2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0)) *
+            size_t(c2r2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0) + size_t(N)) *
            elem;
+}
+
+template <typename T, bool INV>
+__device__ __forceinline__ cplx<T> ltwiddle(const cplx<T> *TW, int j) {
+    cplx<T> w = TW[j];
+    if (INV) w.y = -w.y;
+    return w;
 }
 
 // lengths with instantiated fused transforms (larger planes do not fit in LDS)
@@ -46,7 +55,10 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;
+    cplx<T> *TW = X + P * PS;
     const size_t P2 = size_t(K1) * K2;
+    // the twiddle table, loaded together with the first stage's data
+    for (int j = threadIdx.x; j < N; j += blockDim.x) TW[j] = gload(S.tw + S.p2.tw + j);
     const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
 
     // x2, stage 1: item (c, t), c fastest; A values at k2 positions Bq·n + t
@@ -112,7 +124,6 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     }
     __syncthreads();
     // x1: rounds of P row pairs, Hermitian completion of (row 2p) + i·(row 2p+1)
-    const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
     T *out = S.in_phys[term] + size_t(layer) * N * N;
     for (int p0 = 0; p0 < NP; p0 += P) {
         for (int it = threadIdx.x; it < P * Bq; it += blockDim.x) {
@@ -134,7 +145,7 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
             rdft<T, true, A>(v);
             static_for<0, A>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw1, t * k));
+                X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, t * k));
             });
         }
         __syncthreads();
@@ -180,9 +191,11 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;
+    cplx<T> *TW = X + P * PS;
     const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
-    const cplx<T> *__restrict__ tw2 = S.tw + S.p2.tw;
     const int tid = threadIdx.x;
+    // the twiddle table for the x2 stages (visible after the first barrier)
+    for (int j = tid; j < N; j += blockDim.x) TW[j] = gload(S.tw + S.p2.tw + j);
     const T scale = T(1.0 / (double(N) * double(N)));
     cplx<T> *out = S.tgt_out[tg] + size_t(layer) * K1 * K2;
 
@@ -262,7 +275,7 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
             rdft<T, false, A>(v);
             static_for<0, A>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                Z[(t1 * A + k) * zs + c1] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, false>(tw2, t1 * k));
+                Z[(t1 * A + k) * zs + c1] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, false>(TW, t1 * k));
             });
         }
         __syncthreads();

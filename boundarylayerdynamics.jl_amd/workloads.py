@@ -109,7 +109,7 @@ def c2r_fused(N1, N2, nk1, precision):
     zs = nk1 + ((N // A) // 2 - nk1) % 16
     pairs = 32 if N > 128 else min(N // 2, 64)
     elem = 8 if precision == "fp32" else 16
-    return (N * zs + pairs * (N // A) * (A + 1)) * elem <= 160 * 1024
+    return (N * zs + pairs * (N // A) * (A + 1) + N) * elem <= 160 * 1024
 
 
 def r2c_fused(N1, N2, nk1, precision):
