@@ -295,7 +295,7 @@ class Engine : public EngineBase {
     // r2c_x2 of the SGS spaces is separate (it accumulates into the same rates)
     std::vector<Pass> tp[4];
     std::vector<Pass> tp_r2c_x2_sgs;
-    std::vector<Pass> tp_c2r_2d;  // fused c2r of whole planes (square grids that fit in LDS)
+    std::vector<Pass> tp_c2r2d[2][2];  // fused c2r of whole planes: [advection, SGS][1024-, 512-thread blocks]
     std::vector<Pass> tp_r2c_2d, tp_r2c_2d_sgs;  // fused r2c (advection, SGS: both accumulate into du)
     bool fused_c2r = true, fused_r2c = true;
     int tuned_mode = 3;
@@ -682,26 +682,29 @@ class Engine : public EngineBase {
         return c2r2d_lds(N, S.nk1, sizeof(cplx<T>)) <= 160 * 1024;
     }
 
-    // the fused r2c in addition keeps one x2 item of each stage per thread
+    // the fused r2c: its own LDS layout, one x2 item of each stage per thread
     static bool r2c_2d_fits(const Space<T> &S) {
-        if (!c2r_2d_fits(S)) return false;
+        const int N = S.N2;
+        if (S.N1 != N || !c2r2d_supported<T>(N)) return false;
+        if (r2c2d_lds(N, S.nk1, sizeof(cplx<T>)) > 160 * 1024) return false;
         const int A = reg_fft_a(S.N2), Bq = S.N2 / A;
         return S.nk1 * std::max(A, Bq) <= kC2r2dThreads;
     }
 
     void add_entry(Pass &ps, int kind, int sidx, int batch = 0) {
         const Space<T> &S = spaces[sidx];
-        if (kind == PASS_C2R_2D || kind == PASS_R2C_2D) {  // one block per plane
+        if (kind == PASS_C2R_2D || kind == PASS_C2R_2D_HALF || kind == PASS_R2C_2D) {  // one block per plane
             LaunchEntry E{};
             E.space = sidx;
             E.batch = 1;
             E.blocks_per_job = 1;
-            E.njobs = kind == PASS_C2R_2D ? S.in_job0[S.nin] : S.tgt_job0[S.ntgt];
+            E.njobs = kind == PASS_R2C_2D ? S.tgt_job0[S.ntgt] : S.in_job0[S.nin];
             E.block_begin = ps.nblocks;
             ps.ent.push_back(E);
             ps.kind.push_back(kind);
             ps.nblocks += E.njobs;
-            ps.smem = std::max(ps.smem, c2r2d_lds(S.N2, S.nk1, sizeof(cplx<T>)));
+            ps.smem = std::max(ps.smem, kind == PASS_R2C_2D ? r2c2d_lds(S.N2, S.nk1, sizeof(cplx<T>))
+                                                            : c2r2d_lds(S.N2, S.nk1, sizeof(cplx<T>)));
             return;
         }
         int n, bufs, avail, njobs;
@@ -760,7 +763,8 @@ class Engine : public EngineBase {
     void build_transform_passes(const std::vector<std::vector<int>> *batches = nullptr) {
         for (int k = 0; k < 4; ++k) tp[k].clear();
         tp_r2c_x2_sgs.clear();
-        tp_c2r_2d.clear();
+        for (auto &g : tp_c2r2d)
+            for (auto &v : g) v.clear();
         tp_r2c_2d.clear();
         tp_r2c_2d_sgs.clear();
         for (int k = 0; k < 5; ++k) tp_entry_space[k].clear();
@@ -773,7 +777,8 @@ class Engine : public EngineBase {
             auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
             const bool adv = spaces[s].kind == SK_ADVECTION;
             if (fused_c2r && c2r_2d_fits(spaces[s])) {
-                bucket_add(tp_c2r_2d, PASS_C2R_2D, s, 0);
+                const bool half = c2r2d_threads(spaces[s].N2, spaces[s].nk1, sizeof(cplx<T>)) == 512;
+                bucket_add(tp_c2r2d[adv ? 0 : 1][half ? 1 : 0], half ? PASS_C2R_2D_HALF : PASS_C2R_2D, s, 0);
             } else {
                 for (int k = 0; k < 2; ++k) bucket_add(tp[k], k, s, b(k));
             }
@@ -791,10 +796,15 @@ class Engine : public EngineBase {
         }
         for (int k = 0; k < 4; ++k) bucket_finish(tp[k]);
         bucket_finish(tp_r2c_x2_sgs);
-        bucket_finish(tp_c2r_2d);
+        bool any2d = false;
+        for (auto &g : tp_c2r2d)
+            for (auto &v : g) {
+                bucket_finish(v);
+                any2d |= !v.empty();
+            }
         bucket_finish(tp_r2c_2d);
         bucket_finish(tp_r2c_2d_sgs);
-        if (!tp_c2r_2d.empty() || !tp_r2c_2d.empty() || !tp_r2c_2d_sgs.empty()) HIPCHECK(allow_large_lds<T>());
+        if (any2d || !tp_r2c_2d.empty() || !tp_r2c_2d_sgs.empty()) HIPCHECK(allow_large_lds<T>());
     }
 
     void build_plans() {
@@ -931,32 +941,49 @@ class Engine : public EngineBase {
     }
 
     // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du.
-    // Independent work overlaps on the side stream: the spectral linear terms
-    // with the c2r passes, the advection products with the SGS kernels; both
-    // join before the r2c passes (which read the products and accumulate into
-    // du).  With per-kernel profiling everything stays on the engine stream.
+    // The advection chain (linear terms, whole-plane c2r of the advection
+    // spaces, advection products, whole-plane r2c of their rates) runs on the
+    // side stream while the SGS chain runs on the engine stream; the linear
+    // terms write du before the advection rates accumulate behind them on the
+    // same stream, and the SGS rates accumulate after the join.  Two-pass
+    // transforms cover both kinds of space in one launch, so the side stream
+    // waits for them.  With per-kernel profiling everything stays on the
+    // engine stream.
     void rhs(int sel) {
-        const bool any_c2r = !tp[PASS_C2R_X2].empty() || !tp_c2r_2d.empty();
+        const bool two_pass_c2r = !tp[PASS_C2R_X2].empty();
+        bool any_2d = false;
+        for (auto &g : tp_c2r2d)
+            for (auto &v : g) any_2d |= !v.empty();
+        const bool any_c2r = two_pass_c2r || any_2d;
         const bool side = !profiling && any_c2r;
         const Space<T> *dsp = d_spaces.as<Space<T>>();
+        auto c2r2d = [&](int group, hipStream_t s) {
+            for (int h = 0; h < 2; ++h)
+                for (const Pass &p : tp_c2r2d[group][h]) {
+                    if (s == st) {
+                        run_transform(h ? PASS_C2R_2D_HALF : PASS_C2R_2D, p, sel, KK_C2R_2D);
+                    } else {
+                        launch_transform<T>(h ? PASS_C2R_2D_HALF : PASS_C2R_2D, dsp, p.lt(), p.nblocks, p.smem, sel, s);
+                    }
+                }
+        };
         if (side) {
             fork(ev_fork);
             launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st2);
+            c2r2d(0, st2);
         } else {
             timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
+            c2r2d(0, st);
         }
         if (!any_c2r) return;
-        run_transform(PASS_C2R_2D, tp_c2r_2d, sel, KK_C2R_2D);
+        c2r2d(1, st);
         run_transform(PASS_C2R_X2, tp[PASS_C2R_X2], sel, KK_C2R_X2);
         run_transform(PASS_C2R_X1, tp[PASS_C2R_X1], sel, KK_C2R_X1);
         const int kks[5] = {KK_ADV, KK_SGS_VISC, KK_SGS_STRESS, KK_SGS_MLP, KK_SGS_STRESS};
         bool adv_r2c_done = false;
         if (side && pp[PHYS_ADV].nblocks) {
-            fork(ev_fork2);
+            if (two_pass_c2r) fork(ev_fork2);
             launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st2);
-            // whole-plane r2c of the advection rates: accumulates into du behind
-            // the linear terms on the side stream while the SGS kernels run; the
-            // SGS r2c accumulates after the join
             for (const Pass &p : tp_r2c_2d)
                 launch_transform<T>(PASS_R2C_2D, dsp, p.lt(), p.nblocks, p.smem, sel, st2);
             adv_r2c_done = true;

@@ -113,16 +113,17 @@ __global__ void __launch_bounds__(256) k_c2r_x2(const Space<T> *__restrict__ spa
 #undef BLD_CALL
 }
 
-// fused c2r of whole (term, layer) planes of square grids, one plane per block
-template <typename T>
-__global__ void __launch_bounds__(kC2r2dThreads) k_c2r_2d(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
+// fused c2r of whole (term, layer) planes of square grids, one plane per
+// block of NT threads
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) k_c2r_2d(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
     extern __shared__ unsigned char smem_raw[];
     cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const Space<T> &S = spaces[E.space];
 #define BLD_CALL(n)                                 \
     if constexpr (c2r2d_supported<T>(n))           \
-        c2r_2d_reg<T, n>(S, E, sel, smem)
+        c2r_2d_reg<T, n, NT>(S, E, sel, smem)
     BLD_FFT_SWITCH(S.N2, BLD_CALL)
 #undef BLD_CALL
 }
@@ -1091,7 +1092,8 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
         case PASS_C2R_X1: k_c2r_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_R2C_X1: k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
-        case PASS_C2R_2D: k_c2r_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt, sel); break;
+        case PASS_C2R_2D: k_c2r_2d<T, 1024><<<nblocks, 1024, smem, st>>>(spaces_dev, lt, sel); break;
+        case PASS_C2R_2D_HALF: k_c2r_2d<T, 512><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel); break;
         case PASS_R2C_2D: k_r2c_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt); break;
     }
 }
@@ -1099,8 +1101,11 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
 // the fused transforms keep a whole plane in LDS: allow up to 160 KiB per block
 template <typename T>
 hipError_t allow_large_lds() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T, 1024>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T, 512>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_r2c_2d<T>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

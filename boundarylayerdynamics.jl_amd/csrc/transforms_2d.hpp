@@ -12,21 +12,44 @@
 
 namespace bld {
 
-// LDS layout of the fused transforms: the intermediate Z[N][zs], then the
-// row-pair exchange X[P][B·(A+1)].  The row items of a pair are B consecutive
-// lanes reading B consecutive entries of two Z rows; with zs ≡ B/2 (mod 16)
-// the pairs of a half wave start 2·B banks apart, so their reads of 8-byte
-// entries fall on disjoint banks.
-__host__ __device__ constexpr int c2r2d_zs(int N, int nk1) {
+// Bank spread: the row items of a pair are B consecutive lanes reading B
+// consecutive entries of two Z rows; with a row stride zs ≡ B/2 (mod 16) the
+// pairs of a half wave start 2·B banks apart, so their reads of 8-byte entries
+// fall on disjoint banks.
+__host__ __device__ constexpr int xf2d_spread(int N, int need) {
     const int bh = reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) / 2 : 0;
-    return nk1 + (((bh - nk1) % 16) + 16) % 16;
+    return need + (((bh - need) % 16) + 16) % 16;
 }
-__host__ __device__ constexpr int c2r2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
-// ... followed by the N twiddles w_N^j (one table serves both directions of
-// a square grid), read in the stages whose data come from LDS
+
+// c2r layout: Z[N][zs] and the N twiddles w_N^j (one table serves both
+// directions of a square grid).  Two rows of Z hold the B·(A+1) entries of a
+// row pair's FFT exchange, so the x1 pass works in place in the pair's rows.
+__host__ __device__ constexpr int c2r2d_zs(int N, int nk1) {
+    const int A = reg_fft_a(N) > 0 ? reg_fft_a(N) : 1, Bq = N / A;
+    const int ex = (Bq * (A + 1) + 1) / 2;
+    return xf2d_spread(N, nk1 > ex ? nk1 : ex);
+}
 __host__ __device__ constexpr size_t c2r2d_lds(int N, int nk1, size_t elem) {
-    return (size_t(N) * c2r2d_zs(N, nk1) +
-            size_t(c2r2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0) + size_t(N)) *
+    return (size_t(N) * c2r2d_zs(N, nk1) + size_t(N)) * elem;
+}
+// threads of the c2r block: 512 where two blocks fit in one CU's LDS
+__host__ __device__ constexpr int c2r2d_threads(int N, int nk1, size_t elem) {
+    return c2r2d_lds(N, nk1, elem) <= 80 * 1024 ? 512 : 1024;
+}
+// row pairs per round of the c2r x1 pass: one first-stage item per thread
+__host__ __device__ constexpr int c2r2d_pairs(int N, int NT) {
+    const int A = reg_fft_a(N) > 0 ? reg_fft_a(N) : 1, Bq = N / A, NP = N / 2;
+    const int rounds = (NP * Bq + NT - 1) / NT;
+    return (NP + rounds - 1) / rounds;
+}
+
+// r2c layout: Z[N][zs] (the summed half spectra), the row-pair exchange
+// X[P][B·(A+1)] of one round, the twiddles
+__host__ __device__ constexpr int r2c2d_zs(int N, int nk1) { return xf2d_spread(N, nk1); }
+__host__ __device__ constexpr int r2c2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
+__host__ __device__ constexpr size_t r2c2d_lds(int N, int nk1, size_t elem) {
+    return (size_t(N) * r2c2d_zs(N, nk1) +
+            size_t(r2c2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0) + size_t(N)) *
            elem;
 }
 
@@ -43,10 +66,11 @@ __host__ __device__ constexpr bool c2r2d_supported(int N) {
     return reg_fft_a(N) > 0 && N % 2 == 0 && N <= (sizeof(T) == 4 ? 192 : 96);
 }
 
-template <typename T, int N>
+template <typename T, int N, int NT>
 __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
-    constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1, PS = Bq * RS;
-    constexpr int P = c2r2d_pairs(N), NP = N / 2;
+    constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1;
+    constexpr int P = c2r2d_pairs(N, NT), NP = N / 2;
+    static_assert(P * Bq <= NT, "one first-stage row item per thread");
     const int job = blockIdx.x - E.block_begin;
     int term = 0;
     while (job >= S.in_job0[term + 1]) ++term;
@@ -54,17 +78,16 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     const int recipe = S.in_recipe[term];
     const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
     cplx<T> *Z = smem;
-    cplx<T> *X = smem + size_t(N) * zs;
-    cplx<T> *TW = X + P * PS;
+    cplx<T> *TW = smem + size_t(N) * zs;
     const size_t P2 = size_t(K1) * K2;
     // the twiddle table, loaded together with the first stage's data
-    for (int j = threadIdx.x; j < N; j += blockDim.x) TW[j] = gload(S.tw + S.p2.tw + j);
+    for (int j = threadIdx.x; j < N; j += NT) TW[j] = gload(S.tw + S.p2.tw + j);
     const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
 
     // x2, stage 1: item (c, t), c fastest; A values at k2 positions Bq·n + t
     {
         const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;
-        for (int it = threadIdx.x; it < nk1 * Bq; it += blockDim.x) {
+        for (int it = threadIdx.x; it < nk1 * Bq; it += NT) {
             const int t = it / nk1, c = it - t * nk1;
             int i2s[A];
             bool ok[A];
@@ -107,7 +130,7 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     __syncthreads();
     // x2, stage 2: item (c, t) reads rows n·A + t and writes rows t + A·k, the
     // same set, so the update is in place
-    for (int it = threadIdx.x; it < nk1 * A; it += blockDim.x) {
+    for (int it = threadIdx.x; it < nk1 * A; it += NT) {
         const int t = it / nk1, c = it - t * nk1;
         cplx<T> v[Bq];
         static_for<0, Bq>([&](auto nc) {
@@ -123,51 +146,56 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
         });
     }
     __syncthreads();
-    // x1: rounds of P row pairs, Hermitian completion of (row 2p) + i·(row 2p+1)
+    // x1: rounds of P row pairs, Hermitian completion of (row 2p) + i·(row 2p+1);
+    // the pair's exchange lives in its own two rows (2·zs ≥ Bq·RS entries)
     T *out = S.in_phys[term] + size_t(layer) * N * N;
     for (int p0 = 0; p0 < NP; p0 += P) {
-        for (int it = threadIdx.x; it < P * Bq; it += blockDim.x) {
-            const int pl = it / Bq, t = it - pl * Bq;
-            const int p = min(p0 + pl, NP - 1);
-            const cplx<T> *ra = Z + size_t(2 * p) * zs, *rb = ra + zs;
-            cplx<T> v[A];
-            static_for<0, A>([&](auto nc) {
-                constexpr int n = decltype(nc)::value;
-                const int k = Bq * n + t;
-                const bool direct = k <= k1m, mirror = !direct && N - k <= k1m;
-                const bool real_only = k == 0 || (2 * k == N && direct);
-                const int kk = direct ? k : (mirror ? N - k : 0);
-                const cplx<T> a = ra[kk], b = rb[kk];
-                cplx<T> z = real_only ? mk(a.x, b.x) : direct ? mk(a.x - b.y, a.y + b.x) : mk(a.x + b.y, b.x - a.y);
-                if (!(direct || mirror)) z = czero<T>();
-                v[n] = z;
-            });
+        const int it = threadIdx.x;
+        const int pl = it / Bq, t = it - pl * Bq;
+        const bool own = pl < P && p0 + pl < NP;
+        const int p = min(p0 + pl, NP - 1);
+        cplx<T> *R = Z + size_t(2 * p) * zs;  // rows 2p and 2p+1: the pair's region
+        cplx<T> v[A];
+        static_for<0, A>([&](auto nc) {
+            constexpr int n = decltype(nc)::value;
+            const int k = Bq * n + t;
+            const bool direct = k <= k1m, mirror = !direct && N - k <= k1m;
+            const bool real_only = k == 0 || (2 * k == N && direct);
+            const int kk = direct ? k : (mirror ? N - k : 0);
+            const cplx<T> a = R[kk], b = R[zs + kk];
+            cplx<T> z = real_only ? mk(a.x, b.x) : direct ? mk(a.x - b.y, a.y + b.x) : mk(a.x + b.y, b.x - a.y);
+            if (!(direct || mirror)) z = czero<T>();
+            v[n] = z;
+        });
+        __syncthreads();  // every read of the round's rows is done before the exchange overwrites them
+        if (own) {
             rdft<T, true, A>(v);
             static_for<0, A>([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
-                X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, t * k));
+                R[t * RS + k] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, t * k));
             });
         }
         __syncthreads();
-        for (int it = threadIdx.x; it < P * A; it += blockDim.x) {
-            const int pl = it / A, t = it - pl * A;
-            const int p = p0 + pl;
-            cplx<T> v[Bq];
-            static_for<0, Bq>([&](auto nc) {
-                constexpr int n = decltype(nc)::value;
-                v[n] = X[pl * PS + n * RS + t];
-            });
-            rdft<T, true, Bq>(v);
-            if (p < NP) {
-                T *ra = out + size_t(2 * p) * N, *rb = ra + N;
+        for (int i2 = threadIdx.x; i2 < P * A; i2 += NT) {
+            const int ql = i2 / A, s = i2 - ql * A;
+            const int q = p0 + ql;
+            if (q < NP) {
+                const cplx<T> *Q = Z + size_t(2 * q) * zs;
+                cplx<T> y[Bq];
+                static_for<0, Bq>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    y[n] = Q[n * RS + s];
+                });
+                rdft<T, true, Bq>(y);
+                T *ra = out + size_t(2 * q) * N, *rb = ra + N;
                 static_for<0, Bq>([&](auto kc) {
                     constexpr int k = decltype(kc)::value;
-                    gstore(ra + t + A * k, v[k].x);
-                    gstore(rb + t + A * k, v[k].y);
+                    gstore(ra + s + A * k, y[k].x);
+                    gstore(rb + s + A * k, y[k].y);
                 });
             }
         }
-        __syncthreads();
+        // the next round touches other rows only: no barrier
     }
 }
 
@@ -182,12 +210,12 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
 template <typename T, int N>
 __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
     constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1, PS = Bq * RS;
-    constexpr int P = c2r2d_pairs(N), NP = N / 2;
+    constexpr int P = r2c2d_pairs(N), NP = N / 2;
     const int job = blockIdx.x - E.block_begin;
     int tg = 0;
     while (job >= S.tgt_job0[tg + 1]) ++tg;
     const int layer = job - S.tgt_job0[tg];
-    const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    const int nk1 = S.nk1, zs = r2c2d_zs(N, nk1), k2m = S.k2m, K1 = S.K1, K2 = S.K2;
     const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;

@@ -97,27 +97,40 @@ def spectral_noise(model, seed=0, amp=0.25, k0=3.0):
 REG_FFT_A = {16: 4, 24: 6, 32: 8, 48: 8, 64: 8, 72: 9, 96: 12, 128: 16, 144: 12, 192: 16, 256: 16, 384: 24}
 
 
+def _spread(N, need):
+    bh = (N // REG_FFT_A[N]) // 2
+    return need + (bh - need) % 16
+
+
+def _supported(N1, N2, precision):
+    N = N2
+    return N1 == N2 and N % 2 == 0 and N in REG_FFT_A and N <= (192 if precision == "fp32" else 96)
+
+
 def c2r_fused(N1, N2, nk1, precision):
     """Whether the engine transforms a space's planes with the fused c2r
-    kernel: square grid, even length with a register FFT, at most 192 (fp32)
-    or 96 (fp64) points, and the plane's intermediate plus one round of row
-    pairs within 160 KiB of LDS (the engine's own rule)."""
-    N = N2
-    if N1 != N2 or N % 2 or N not in REG_FFT_A or N > (192 if precision == "fp32" else 96):
+    kernel (the engine's own rule): square grid, even length with a register
+    FFT, at most 192 (fp32) or 96 (fp64) points, and the plane's intermediate
+    (rows wide enough for a row pair's exchange) plus the twiddles within
+    160 KiB of LDS."""
+    if not _supported(N1, N2, precision):
         return False
-    A = REG_FFT_A[N]
-    zs = nk1 + ((N // A) // 2 - nk1) % 16
-    pairs = 32 if N > 128 else min(N // 2, 64)
+    N, A = N2, REG_FFT_A[N2]
+    zs = _spread(N, max(nk1, ((N // A) * (A + 1) + 1) // 2))
     elem = 8 if precision == "fp32" else 16
-    return (N * zs + pairs * (N // A) * (A + 1) + N) * elem <= 160 * 1024
+    return (N * zs + N) * elem <= 160 * 1024
 
 
 def r2c_fused(N1, N2, nk1, precision):
-    """The fused r2c in addition keeps one x2 item per thread (1024 threads)."""
-    if not c2r_fused(N1, N2, nk1, precision):
+    """The fused r2c: its intermediate, one round of row-pair exchange and the
+    twiddles within 160 KiB of LDS, one x2 item per thread (1024 threads)."""
+    if not _supported(N1, N2, precision):
         return False
-    A = REG_FFT_A[N2]
-    return nk1 * max(A, N2 // A) <= 1024
+    N, A = N2, REG_FFT_A[N2]
+    pairs = 32 if N > 128 else min(N // 2, 64)
+    elem = 8 if precision == "fp32" else 16
+    lds = (N * _spread(N, nk1) + pairs * (N // A) * (A + 1) + N) * elem
+    return lds <= 160 * 1024 and nk1 * max(A, N // A) <= 1024
 
 
 def algorithmic_bytes(cfg_sizes, kind, precision):
