@@ -224,6 +224,7 @@ def closure_bench(B, res, dt, steps, warmup):
     m.set_physical("vel2", v)
     m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
     m.init_problem()
+    m.autotune(reps=3)
     m.step(dt, warmup)
     ms = m.time_steps(dt, steps)
     m.set_profiling(True)
@@ -238,6 +239,7 @@ def closure_bench(B, res, dt, steps, warmup):
     npts = 1 << 20
     f_ms, g_ms = net.time(npts, reps=10)
     return dict(steps_per_s=round(1e3 * steps / ms, 2), ms_per_step=round(ms / steps, 4), finite=finite,
+                transform_path=m.tuning()["transform_path"],
                 sgs_mlp_avg_us=round(1e3 * ms_mlp / launches, 2), points_per_launch=pts,
                 sgs_mlp_tflops=round(tflops, 2), sgs_mlp_frac_bf16_peak=round(tflops / BF16_DENSE_PEAK_TFLOPS, 4),
                 kernels={k: round(1e3 * v[0] / v[1], 2) for k, v in kt.items() if v[1]},

@@ -246,6 +246,7 @@ class Engine : public EngineBase {
         HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_lin, hipEventDisableTiming));
         d_flag.alloc(sizeof(int));
         HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
         for (int i = 0; i < n; ++i) {
@@ -266,6 +267,7 @@ class Engine : public EngineBase {
         if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_fork2) (void)hipEventDestroy(ev_fork2);
         if (ev_join2) (void)hipEventDestroy(ev_join2);
+        if (ev_lin) (void)hipEventDestroy(ev_lin);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -274,7 +276,7 @@ class Engine : public EngineBase {
     // side stream of the float64 (0,0)-mode stage pass, forked from and joined
     // back into `st` (also inside the captured graph)
     hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr, ev_lin = nullptr;
     std::vector<std::unique_ptr<Instance<T>>> inst;
     int sel_inst = 0;
     TwiddleSet<T> tw;
@@ -941,11 +943,11 @@ class Engine : public EngineBase {
     }
 
     // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du.
-    // The advection chain (linear terms, whole-plane c2r of the advection
-    // spaces, advection products, whole-plane r2c of their rates) runs on the
-    // side stream while the SGS chain runs on the engine stream; the linear
-    // terms write du before the advection rates accumulate behind them on the
-    // same stream, and the SGS rates accumulate after the join.  Two-pass
+    // The advection chain (whole-plane c2r of the advection spaces, advection
+    // products, whole-plane r2c of their rates) runs on the side stream while
+    // the linear terms and the SGS chain run on the engine stream; the
+    // advection rates wait for the linear terms' write of du (an event) before
+    // they accumulate, and the SGS rates accumulate after the join.  Two-pass
     // transforms cover both kinds of space in one launch, so the side stream
     // waits for them.  With per-kernel profiling everything stays on the
     // engine stream.
@@ -969,8 +971,9 @@ class Engine : public EngineBase {
         };
         if (side) {
             fork(ev_fork);
-            launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st2);
             c2r2d(0, st2);
+            launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st);
+            HIPCHECK(hipEventRecord(ev_lin, st));
         } else {
             timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
             c2r2d(0, st);
@@ -984,6 +987,7 @@ class Engine : public EngineBase {
         if (side && pp[PHYS_ADV].nblocks) {
             if (two_pass_c2r) fork(ev_fork2);
             launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st2);
+            HIPCHECK(hipStreamWaitEvent(st2, ev_lin, 0));  // du holds the linear terms
             for (const Pass &p : tp_r2c_2d)
                 launch_transform<T>(PASS_R2C_2D, dsp, p.lt(), p.nblocks, p.smem, sel, st2);
             adv_r2c_done = true;
@@ -1397,9 +1401,9 @@ class Engine : public EngineBase {
             }
             fused_c2r = (bm & 1) != 0;
             fused_r2c = (bm & 2) != 0;
+            build_transform_passes(&best);
             tuned_mode = bm;
             tuned_us_rhs = bt;
-            build_transform_passes(&best);
         }
         // chunk length of the stage kernel: stage 1 only writes utmp, so it can be
         // replayed on the live state
