@@ -344,18 +344,22 @@ TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_les128x128x64_fp32.md")
 
 
 def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (the 'traffic MB (scaled)' column), or None."""
+    """HBM bytes per dispatch of kernel class `kernel` (float32 kernels) from the
+    committed counter summary (the 'traffic MB (scaled)' column): the mean over
+    the class's kernels weighted by their dispatches (a class such as c2r_2d has
+    one kernel per block size), or None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
+    tot, n = 0.0, 0
     for line in open(TRAFFIC_FILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith("k_" + kernel + "<"):
+        if len(cells) == 5 and cells[0].startswith("k_" + kernel + "<float"):
             try:
-                return int(float(cells[4]) * 1e6)
+                tot += float(cells[4]) * 1e6 * int(cells[1])
+                n += int(cells[1])
             except ValueError:
                 return None
-    return None
+    return int(tot / n) if n else None
 
 
 def main():
