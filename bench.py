@@ -64,6 +64,10 @@ def time_form(B, precision, res, dt, steps, warmup, profile_steps, autotune=Fals
     wall = time.perf_counter() - t0
     kt = None
     if profile_steps:
+        # an idle gap before the serialized profiled steps, so that a kernel
+        # trace of this run can tell them from the graph replays (which run
+        # the two streams concurrently)
+        time.sleep(PROFILE_GAP_S)
         m.set_profiling(True)
         m.step(dt, profile_steps, graph=False)
         kt = m.kernel_times()
@@ -298,6 +302,21 @@ def _counter_per_kernel(directory, counter):
     return out
 
 
+PROFILE_GAP_S = 0.05
+
+
+def _after_last_gap(trace, gap_ns):
+    """Dispatches after the last idle gap of at least gap_ns in the trace."""
+    ev = sorted(trace, key=lambda r: int(r["Start_Timestamp"]))
+    end, cut = None, 0
+    for i, r in enumerate(ev):
+        s0 = int(r["Start_Timestamp"])
+        if end is not None and s0 - end >= gap_ns:
+            cut = i
+        end = max(end or 0, int(r["End_Timestamp"]))
+    return ev[cut:]
+
+
 def rocprof_summary(kt_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_bytes):
     """Markdown summary of a rocprofv3 --kernel-trace --stats run and of the
     FETCH_SIZE / WRITE_SIZE runs, the counters scaled by what they read for the
@@ -322,6 +341,17 @@ def rocprof_summary(kt_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir
         d = [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in late]
         grids = sorted({int(grid_of(r)) for r in late})
         lines.append(f"| {k} | {len(d)} | {' '.join(map(str, grids))} | {sum(d) / len(d) / 1e3:.2f} |")
+    # the serialized profiled steps that follow bench.py's idle gap: the
+    # per-dispatch durations behind bench.py's kernel figures and roofline
+    prof = _after_last_gap(trace, int(0.8 * PROFILE_GAP_S * 1e9))
+    byp = {}
+    for r in prof:
+        byp.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    lines += ["", "Profiled steps (serialized on one stream, after bench.py's idle gap; the dispatches "
+                  "bench.py's per-kernel averages and roofline come from):", "",
+              "| kernel | dispatches | average us |", "|---|---|---|"]
+    for k, d in sorted(byp.items(), key=lambda kv: -sum(kv[1])):
+        lines.append(f"| {k} | {len(d)} | {sum(d) / len(d) / 1e3:.2f} |")
     cf = _counter_per_kernel(copy_fetch_dir, "FETCH_SIZE")
     cw = _counter_per_kernel(copy_write_dir, "WRITE_SIZE")
     ck = max(cf, key=lambda k: cf[k][0])

@@ -44,12 +44,13 @@ __host__ __device__ constexpr int c2r2d_pairs(int N, int NT) {
 }
 
 // r2c layout: Z[N][zs] (the summed half spectra), the row-pair exchange
-// X[P][B·(A+1)] of one round, the twiddles
+// X[P][B·(A+1)] of one round, the twiddles, the x1 derivative factors D1[k1]
 __host__ __device__ constexpr int r2c2d_zs(int N, int nk1) { return xf2d_spread(N, nk1); }
 __host__ __device__ constexpr int r2c2d_pairs(int N) { return N > 128 ? 32 : (N / 2 < 64 ? N / 2 : 64); }
 __host__ __device__ constexpr size_t r2c2d_lds(int N, int nk1, size_t elem) {
     return (size_t(N) * r2c2d_zs(N, nk1) +
-            size_t(r2c2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0) + size_t(N)) *
+            size_t(r2c2d_pairs(N)) * (reg_fft_a(N) > 0 ? (N / reg_fft_a(N)) * (reg_fft_a(N) + 1) : 0) + size_t(N) +
+            size_t(nk1)) *
            elem;
 }
 
@@ -220,10 +221,13 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;
     cplx<T> *TW = X + P * PS;
+    cplx<T> *DF = TW + N;  // D1[k1] in .x
     const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
     const int tid = threadIdx.x;
-    // the twiddle table for the x2 stages (visible after the first barrier)
+    // the twiddle table for the x2 stages and the D1 factors of the row split
+    // (visible after the first barrier)
     for (int j = tid; j < N; j += blockDim.x) TW[j] = gload(S.tw + S.p2.tw + j);
+    for (int j = tid; j < nk1; j += blockDim.x) DF[j] = mk(T(gload(S.D1 + j)), T(0));
     const T scale = T(1.0 / (double(N) * double(N)));
     cplx<T> *out = S.tgt_out[tg] + size_t(layer) * K1 * K2;
 
@@ -274,7 +278,7 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
                     cplx<T> ra = mk(T(0.5) * (zk.x + zn.x), T(0.5) * (zk.y - zn.y));
                     cplx<T> rb = mk(T(0.5) * (zk.y + zn.y), T(-0.5) * (zk.x - zn.x));
                     if (dx) {
-                        const T d1 = T(gload(S.D1 + k));
+                        const T d1 = DF[k].x;
                         ra = imul(d1, ra);
                         rb = imul(d1, rb);
                     }
@@ -291,6 +295,18 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     auto columns = [&](bool d2) {
         const bool own1 = tid < nk1 * Bq;
         const int t1 = tid / nk1, c1 = tid - t1 * nk1;
+        const int t2 = tid / nk1, c2 = tid - t2 * nk1;
+        const bool own2 = tid < nk1 * A;
+        // the target values this thread updates, loaded before the transform
+        // so that their latency hides behind it (resolved modes only)
+        cplx<T> old[Bq];
+        static_for<0, Bq>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int pos = t2 + A * k;
+            const bool lo = pos <= k2m, hi = pos >= N - k2m;
+            const int i2 = lo ? pos : (hi ? K2 - (N - pos) : 0);
+            old[k] = own2 ? gload(out + size_t(i2) * K1 + (own2 ? c2 : 0)) : czero<T>();
+        });
         cplx<T> v[A];
         if (own1) {
             static_for<0, A>([&](auto nc) {
@@ -307,8 +323,7 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
             });
         }
         __syncthreads();
-        const int t2 = tid / nk1, c2 = tid - t2 * nk1;
-        if (tid < nk1 * A) {
+        if (own2) {
             cplx<T> w[Bq];
             static_for<0, Bq>([&](auto nc) {
                 constexpr int n = decltype(nc)::value;
@@ -322,8 +337,7 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
                 if (lo || hi) {
                     const int i2 = lo ? pos : K2 - (N - pos);
                     const cplx<T> val = d2 ? imul(T(gload(S.D2 + i2)), w[k]) : w[k];
-                    const size_t o = size_t(i2) * K1 + c2;
-                    gstore(out + o, gload(out + o) + scale * val);
+                    gstore(out + size_t(i2) * K1 + c2, old[k] + scale * val);
                 }
             });
         }

@@ -129,7 +129,7 @@ def r2c_fused(N1, N2, nk1, precision):
     N, A = N2, REG_FFT_A[N2]
     pairs = 32 if N > 128 else min(N // 2, 64)
     elem = 8 if precision == "fp32" else 16
-    lds = (N * _spread(N, nk1) + pairs * (N // A) * (A + 1) + N) * elem
+    lds = (N * _spread(N, nk1) + pairs * (N // A) * (A + 1) + N + nk1) * elem
     return lds <= 160 * 1024 and nk1 * max(A, N // A) <= 1024
 
 
