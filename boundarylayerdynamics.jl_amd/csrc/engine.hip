@@ -1691,6 +1691,140 @@ __global__ void __launch_bounds__(64) k_ldlt32_ir(const float *__restrict__ gamm
     }
 }
 
+// Chunked batched LDLt solve: each system m is split into NC chunks of CL
+// levels held in registers by NC threads (lanes are consecutive systems, so
+// the loads coalesce); all loads of a chunk are issued before any is used.
+// Both sweeps are affine recurrences, y_l = a_l·y_{l-1} + c_l, run from a zero
+// incoming value per chunk while tracking the product of the a_l, and the
+// true incoming values follow from one combine over the chunks through LDS
+// (pressure.jl:126-160, restated as in the fused stage kernel).  With IR the
+// float32 solution gets one refinement step: the residual b − A·x in float64
+// (dv, β and the chunk-edge values of x from the neighbouring chunks through
+// LDS) is solved the same way and added.
+template <typename A, int CL, bool IR>
+__global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamma, const double *__restrict__ beta,
+                                                      const double *__restrict__ dv, const cplx<A> *b,
+                                                      cplx<A> *x, int nk, int nz, int MB) {  // b may be x
+    extern __shared__ unsigned char smem_raw[];
+    const int NC = blockDim.x / MB;
+    cplx<A> *s_y = reinterpret_cast<cplx<A> *>(smem_raw);  // [NC][MB]
+    A *s_p = reinterpret_cast<A *>(s_y + NC * MB);         // [NC][MB]
+    cplx<A> *s_e = reinterpret_cast<cplx<A> *>(s_p + NC * MB);  // [2][NC][MB] chunk-edge x (IR)
+    const int tx = threadIdx.x % MB, ty = threadIdx.x / MB;
+    const int m = blockIdx.x * MB + tx;
+    const bool active = m < nk;
+    const int mm = active ? m : 0;
+    const int l0 = ty * CL;
+    A g[CL], bl[CL], bu[CL];
+    cplx<A> r[CL];
+#pragma unroll
+    for (int j = 0; j < CL; ++j) {
+        const int lc = min(l0 + j, nz - 1);
+        g[j] = gload(gamma + size_t(lc) * nk + mm);
+        r[j] = gload(b + size_t(lc) * nk + mm);
+        bl[j] = A(gload(beta + size_t(max(lc - 1, 0)) * nk + mm));  // couples l-1 and l
+        bu[j] = A(gload(beta + size_t(min(lc, nz - 2 < 0 ? 0 : nz - 2)) * nk + mm));  // couples l and l+1
+    }
+    auto solve = [&](cplx<A>(&v)[CL]) {
+        // forward sweep from a zero incoming value: y_l = g_l·(v_l − β_{l−1}·y_{l−1})
+        A P[CL];
+        cplx<A> yp = mk(A(0), A(0));
+        A pp = A(1);
+#pragma unroll
+        for (int j = 0; j < CL; ++j) {
+            const int l = l0 + j;
+            if (l < nz) {
+                const A bj = l > 0 ? bl[j] : A(0);
+                v[j] = mk(g[j] * (v[j].x - bj * yp.x), g[j] * (v[j].y - bj * yp.y));
+                P[j] = -(g[j] * bj) * pp;
+                yp = v[j];
+                pp = P[j];
+            } else {
+                P[j] = A(0);
+            }
+        }
+        s_y[ty * MB + tx] = yp;
+        s_p[ty * MB + tx] = pp;
+        __syncthreads();
+        cplx<A> yin = mk(A(0), A(0));
+        for (int c = 0; c < ty; ++c) {
+            const cplx<A> w = s_y[c * MB + tx];
+            const A q = s_p[c * MB + tx];
+            yin = mk(w.x + q * yin.x, w.y + q * yin.y);
+        }
+#pragma unroll
+        for (int j = 0; j < CL; ++j) v[j] = mk(v[j].x + P[j] * yin.x, v[j].y + P[j] * yin.y);
+        __syncthreads();
+        // backward sweep from a zero value above the chunk: x_l = y_l − g_l·β_l·x_{l+1}
+        cplx<A> xn = mk(A(0), A(0));
+        A qn = A(1);
+#pragma unroll
+        for (int j = CL - 1; j >= 0; --j) {
+            const int l = l0 + j;
+            if (l < nz) {
+                if (l == nz - 1) {
+                    P[j] = A(0);
+                } else {
+                    const A gb = g[j] * bu[j];
+                    v[j] = mk(v[j].x - gb * xn.x, v[j].y - gb * xn.y);
+                    P[j] = -gb * qn;
+                }
+                xn = v[j];
+                qn = P[j];
+            } else {
+                P[j] = A(0);
+            }
+        }
+        s_y[ty * MB + tx] = xn;
+        s_p[ty * MB + tx] = qn;
+        __syncthreads();
+        cplx<A> xin = mk(A(0), A(0));
+        for (int c = NC - 1; c > ty; --c) {
+            const cplx<A> w = s_y[c * MB + tx];
+            const A q = s_p[c * MB + tx];
+            xin = mk(w.x + q * xin.x, w.y + q * xin.y);
+        }
+#pragma unroll
+        for (int j = 0; j < CL; ++j) v[j] = mk(v[j].x + P[j] * xin.x, v[j].y + P[j] * xin.y);
+        __syncthreads();
+    };
+    cplx<A> xs[CL];
+#pragma unroll
+    for (int j = 0; j < CL; ++j) xs[j] = r[j];
+    solve(xs);
+    if constexpr (IR) {
+        // residual in float64 with the neighbouring chunks' edge values
+        double d[CL];
+#pragma unroll
+        for (int j = 0; j < CL; ++j) d[j] = gload(dv + size_t(min(l0 + j, nz - 1)) * nk + mm);
+        s_e[ty * MB + tx] = xs[0];
+        s_e[(NC + ty) * MB + tx] = xs[CL - 1];
+        __syncthreads();
+        cplx<A> below = ty > 0 ? s_e[(NC + ty - 1) * MB + tx] : mk(A(0), A(0));
+        cplx<A> above = ty + 1 < NC ? s_e[(ty + 1) * MB + tx] : mk(A(0), A(0));
+        cplx<A> rs[CL];
+#pragma unroll
+        for (int j = 0; j < CL; ++j) {
+            const int l = l0 + j;
+            const cplx<A> xm = j > 0 ? xs[j - 1] : below;
+            const cplx<A> xp = j + 1 < CL ? xs[j + 1] : above;
+            const double blo = l > 0 ? double(gload(beta + size_t(max(l - 1, 0)) * nk + mm)) : 0.0;
+            const double bup = l + 1 < nz ? double(gload(beta + size_t(min(l, max(nz - 2, 0))) * nk + mm)) : 0.0;
+            const double rx = double(r[j].x) - (d[j] * double(xs[j].x) + blo * double(xm.x) + (l + 1 < nz ? bup * double(xp.x) : 0.0));
+            const double ry = double(r[j].y) - (d[j] * double(xs[j].y) + blo * double(xm.y) + (l + 1 < nz ? bup * double(xp.y) : 0.0));
+            rs[j] = mk(A(rx), A(ry));
+        }
+        solve(rs);
+#pragma unroll
+        for (int j = 0; j < CL; ++j) xs[j] = mk(xs[j].x + rs[j].x, xs[j].y + rs[j].y);
+    }
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < CL; ++j)
+            if (l0 + j < nz) gstore(x + size_t(l0 + j) * nk + m, xs[j]);
+    }
+}
+
 struct Ldlt {
     int64_t nk = 0, nz = 0;
     int precision = 0;
@@ -1996,7 +2130,26 @@ void bld_ldlt_destroy(bld_ldlt *s) { delete s; }
 const char *bld_ldlt_last_error(const bld_ldlt *s) { return s ? s->s.err.c_str() : g_create_error.c_str(); }
 
 static void ldlt_run(Ldlt &s, hipStream_t st) {
-    const int nb = int((s.nk + 63) / 64);
+    constexpr int CL = 16;
+    int NC = 1;
+    while (NC * CL < s.nz) NC *= 2;
+    if (NC <= 256) {  // chunked: NC chunks of CL levels per system, MB systems per block
+        const int MB = 256 / NC;
+        const int nbk = int((s.nk + MB - 1) / MB);
+        if (s.precision == BLD_FP64) {
+            const size_t sm = 256 * (sizeof(cplx<double>) + sizeof(double));
+            k_ldlt_chunked<double, CL, false><<<nbk, 256, sm, st>>>(s.gamma.as<double>(), s.beta.as<double>(), nullptr,
+                                                                     s.b64.as<cplx<double>>(), s.b64.as<cplx<double>>(),
+                                                                     int(s.nk), int(s.nz), MB);
+        } else {
+            const size_t sm = 256 * (3 * sizeof(cplx<float>) + sizeof(float));
+            k_ldlt_chunked<float, CL, true><<<nbk, 256, sm, st>>>(s.gamma32.as<float>(), s.beta.as<double>(),
+                                                                   s.dv.as<double>(), s.b32.as<cplx<float>>(),
+                                                                   s.x32.as<cplx<float>>(), int(s.nk), int(s.nz), MB);
+        }
+        return;
+    }
+    const int nb = int((s.nk + 63) / 64);  // very tall systems: one thread per system
     if (s.precision == BLD_FP64)
         k_ldlt64<<<nb, 64, 0, st>>>(s.gamma.as<double>(), s.beta.as<double>(), s.b64.as<cplx<double>>(), int(s.nk),
                                     int(s.nz));

@@ -243,6 +243,23 @@ def test_ldlt_dense(N):
     assert relerr(x, xo) < 1e-14
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+@pytest.mark.parametrize("nz", [17, 300])
+def test_ldlt_tall_systems(precision, tol, nz):
+    """Systems taller than one chunk of levels and not a multiple of it: the
+    chunked solve with its combine over chunks (and, in float32, the
+    refinement step across chunk edges) against the oracle's sequential
+    elimination (pressure.jl:106-201)."""
+    rng = np.random.default_rng(nz)
+    nk = 37
+    dv = -(rng.random((nk, nz)) + 2.5)
+    ev = rng.random((nk, nz - 1))
+    b = rng.random((nk, nz)) + 1j * rng.random((nk, nz))
+    x = B.BatchLDLt(dv, ev, precision=precision).solve(b)
+    xo = O.BatchLDLt(dv, ev).solve(b.copy())
+    assert relerr(x, xo) < tol
+
+
 def pressure_operator(nk1, nk2, nz, L=(2 * np.pi, 2 * np.pi), L3=1.0):
     """The pressure matrix of pressure.jl:231-296 for a uniform grid."""
     K1, K2 = nk1, nk2
