@@ -1747,6 +1747,7 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
         s_p[ty * MB + tx] = pp;
         __syncthreads();
         cplx<A> yin = mk(A(0), A(0));
+#pragma unroll 8
         for (int c = 0; c < ty; ++c) {
             const cplx<A> w = s_y[c * MB + tx];
             const A q = s_p[c * MB + tx];
@@ -1779,6 +1780,7 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
         s_p[ty * MB + tx] = qn;
         __syncthreads();
         cplx<A> xin = mk(A(0), A(0));
+#pragma unroll 8
         for (int c = NC - 1; c > ty; --c) {
             const cplx<A> w = s_y[c * MB + tx];
             const A q = s_p[c * MB + tx];

@@ -959,6 +959,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     s_p[ty * MB + tx] = Pprev;
     __syncthreads();
     cplx<double> yin = mk(0.0, 0.0);
+#pragma unroll 8
     for (int c = 0; c < ty; ++c) {
         const cplx<double> v = s_y[c * MB + tx];
         const double pp = s_p[c * MB + tx];
@@ -991,6 +992,7 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     s_p[ty * MB + tx] = Qnext;
     __syncthreads();
     cplx<double> pin = mk(0.0, 0.0);  // p at the level just above this chunk
+#pragma unroll 8
     for (int c = NC - 1; c > ty; --c) {
         const cplx<double> v = s_y[c * MB + tx];
         const double q = s_p[c * MB + tx];
