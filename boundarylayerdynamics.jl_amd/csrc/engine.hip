@@ -297,7 +297,8 @@ class Engine : public EngineBase {
     // r2c_x2 of the SGS spaces is separate (it accumulates into the same rates)
     std::vector<Pass> tp[4];
     std::vector<Pass> tp_r2c_x2_sgs;
-    std::vector<Pass> tp_c2r2d[2][2];  // fused c2r of whole planes: [advection, SGS][1024-, 512-thread blocks]
+    std::vector<Pass> tp_c2r2d[2][3];  // fused c2r: [advection, SGS][1024-, 512-thread whole-plane, half-plane blocks]
+    bool half_c2r = true;              // half-plane c2r where a whole plane needs 1024-thread blocks
     std::vector<Pass> tp_r2c_2d, tp_r2c_2d_sgs;  // fused r2c (advection, SGS: both accumulate into du)
     bool fused_c2r = true, fused_r2c = true;
     int tuned_mode = 3;
@@ -695,6 +696,19 @@ class Engine : public EngineBase {
 
     void add_entry(Pass &ps, int kind, int sidx, int batch = 0) {
         const Space<T> &S = spaces[sidx];
+        if (kind == PASS_C2R_2H) {  // two blocks per plane: its even and its odd rows
+            LaunchEntry E{};
+            E.space = sidx;
+            E.batch = 1;
+            E.blocks_per_job = 2;
+            E.njobs = S.in_job0[S.nin];
+            E.block_begin = ps.nblocks;
+            ps.ent.push_back(E);
+            ps.kind.push_back(kind);
+            ps.nblocks += 2 * E.njobs;
+            ps.smem = std::max(ps.smem, c2r2h_lds(S.N2, S.nk1, sizeof(cplx<T>)));
+            return;
+        }
         if (kind == PASS_C2R_2D || kind == PASS_C2R_2D_HALF || kind == PASS_R2C_2D) {  // one block per plane
             LaunchEntry E{};
             E.space = sidx;
@@ -779,8 +793,13 @@ class Engine : public EngineBase {
             auto b = [&](int k) { return batches ? (*batches)[k][i] : 0; };
             const bool adv = spaces[s].kind == SK_ADVECTION;
             if (fused_c2r && c2r_2d_fits(spaces[s])) {
-                const bool half = c2r2d_threads(spaces[s].N2, spaces[s].nk1, sizeof(cplx<T>)) == 512;
-                bucket_add(tp_c2r2d[adv ? 0 : 1][half ? 1 : 0], half ? PASS_C2R_2D_HALF : PASS_C2R_2D, s, 0);
+                const Space<T> &S = spaces[s];
+                const bool small = c2r2d_threads(S.N2, S.nk1, sizeof(cplx<T>)) == 512;
+                const bool halves = !small && half_c2r && c2r2h_supported<T>(S.N2) &&
+                                    c2r2h_lds(S.N2, S.nk1, sizeof(cplx<T>)) <= 80 * 1024;
+                const int v = small ? 1 : (halves ? 2 : 0);
+                const int kind = small ? PASS_C2R_2D_HALF : (halves ? PASS_C2R_2H : PASS_C2R_2D);
+                bucket_add(tp_c2r2d[adv ? 0 : 1][v], kind, s, 0);
             } else {
                 for (int k = 0; k < 2; ++k) bucket_add(tp[k], k, s, b(k));
             }
@@ -960,12 +979,13 @@ class Engine : public EngineBase {
         const bool side = !profiling && any_c2r;
         const Space<T> *dsp = d_spaces.as<Space<T>>();
         auto c2r2d = [&](int group, hipStream_t s) {
-            for (int h = 0; h < 2; ++h)
+            const int kinds[3] = {PASS_C2R_2D, PASS_C2R_2D_HALF, PASS_C2R_2H};
+            for (int h = 0; h < 3; ++h)
                 for (const Pass &p : tp_c2r2d[group][h]) {
                     if (s == st) {
-                        run_transform(h ? PASS_C2R_2D_HALF : PASS_C2R_2D, p, sel, KK_C2R_2D);
+                        run_transform(kinds[h], p, sel, KK_C2R_2D);
                     } else {
-                        launch_transform<T>(h ? PASS_C2R_2D_HALF : PASS_C2R_2D, dsp, p.lt(), p.nblocks, p.smem, sel, s);
+                        launch_transform<T>(kinds[h], dsp, p.lt(), p.nblocks, p.smem, sel, s);
                     }
                 }
         };
@@ -1386,6 +1406,7 @@ class Engine : public EngineBase {
         // bit 1: fused r2c): whole right-hand sides on the live state with each
         // choice, the fastest kept
         {
+            half_c2r = true;
             double bt = 1e30;
             int bm = 3;
             for (int mode = 3; mode >= 0; --mode) {
@@ -1402,6 +1423,26 @@ class Engine : public EngineBase {
             fused_c2r = (bm & 1) != 0;
             fused_r2c = (bm & 2) != 0;
             build_transform_passes(&best);
+            // half-plane blocks for the planes too large for two whole-plane blocks per CU (bit 2)
+            bool any_half = false;
+            for (auto &g : tp_c2r2d) any_half |= !g[2].empty();
+            if (any_half) {
+                double bh_us = 1e30;
+                int bh = 1;
+                for (int h = 1; h >= 0; --h) {
+                    half_c2r = h == 1;
+                    build_transform_passes(&best);
+                    const double us = time_us([&] { rhs(0); }, reps);
+                    if (us < bh_us) {
+                        bh_us = us;
+                        bh = h;
+                    }
+                }
+                half_c2r = bh == 1;
+                bt = bh_us;
+                bm = (bm & 3) | (bh << 2);
+                build_transform_passes(&best);
+            }
             tuned_mode = bm;
             tuned_us_rhs = bt;
         }
@@ -1438,8 +1479,10 @@ class Engine : public EngineBase {
             *us = tuned_us[4].empty() ? 0.0 : tuned_us[4][0];
             return;
         }
-        if (pass == 5) {  // transform path: bit 0 fused c2r, bit 1 fused r2c; one right-hand side's time
-            *batch = (fused_c2r ? 1 : 0) | (fused_r2c ? 2 : 0);
+        if (pass == 5) {  // bit 0 fused c2r, bit 1 fused r2c, bit 2 half-plane c2r blocks; one rhs time
+            bool any_half = false;
+            for (auto &g : tp_c2r2d) any_half |= !g[2].empty();
+            *batch = (fused_c2r ? 1 : 0) | (fused_r2c ? 2 : 0) | (any_half ? 4 : 0);
             *us = tuned_us_rhs;
             return;
         }

@@ -128,6 +128,21 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) k_
 #undef BLD_CALL
 }
 
+// half-plane c2r: blocks 2j and 2j+1 produce the even and the odd rows of plane j
+template <typename T>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+k_c2r_2h(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<T> *smem = reinterpret_cast<cplx<T> *>(smem_raw);
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+#define BLD_CALL(n)                           \
+    if constexpr (c2r2h_supported<T>(n))     \
+        c2r_2h_reg<T, n, 512>(S, E, sel, smem)
+    BLD_FFT_SWITCH(S.N2, BLD_CALL)
+#undef BLD_CALL
+}
+
 // fused r2c of whole (target, layer) planes of square grids, one plane per block
 template <typename T>
 __global__ void __launch_bounds__(kC2r2dThreads) k_r2c_2d(const Space<T> *__restrict__ spaces, LaunchTable lt) {
@@ -1096,6 +1111,7 @@ void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &l
         case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
         case PASS_C2R_2D: k_c2r_2d<T, 1024><<<nblocks, 1024, smem, st>>>(spaces_dev, lt, sel); break;
         case PASS_C2R_2D_HALF: k_c2r_2d<T, 512><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel); break;
+        case PASS_C2R_2H: k_c2r_2h<T><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel); break;
         case PASS_R2C_2D: k_r2c_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt); break;
     }
 }
@@ -1107,6 +1123,9 @@ hipError_t allow_large_lds() {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T, 512>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2h<T>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_r2c_2d<T>),

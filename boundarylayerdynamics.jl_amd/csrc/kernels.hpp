@@ -12,6 +12,7 @@ enum PassKind : int {
     PASS_C2R_2D = 4,       // fused c2r, 1024-thread blocks
     PASS_R2C_2D = 5,       // fused r2c, 1024-thread blocks
     PASS_C2R_2D_HALF = 6,  // fused c2r, 512-thread blocks (two per CU)
+    PASS_C2R_2H = 7,       // half-plane c2r (even / odd rows), 512-thread blocks (two per CU)
 };
 constexpr int kC2r2dThreads = 1024;  // block of the fused r2c (one plane per block)
 enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2, PHYS_SGS_MLP = 3, PHYS_SGS_MLP_STRESS = 4 };

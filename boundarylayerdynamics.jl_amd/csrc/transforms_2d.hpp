@@ -67,6 +67,20 @@ __host__ __device__ constexpr bool c2r2d_supported(int N) {
     return reg_fft_a(N) > 0 && N % 2 == 0 && N <= (sizeof(T) == 4 ? 192 : 96);
 }
 
+// Half-plane c2r (rows of one parity r of a plane of length N): Z holds the
+// N/2 rows of that parity with the same row stride rule as the whole plane
+__host__ __device__ constexpr int c2r2h_zs(int N, int nk1) { return c2r2d_zs(N, nk1); }
+__host__ __device__ constexpr size_t c2r2h_lds(int N, int nk1, size_t elem) {
+    return (size_t(N / 2) * c2r2h_zs(N, nk1) + size_t(N)) * elem;
+}
+// lengths with an instantiated half-plane c2r: N/2 with a register FFT, and a
+// whole plane too large for two blocks per CU (N ≥ 144 in float32)
+template <typename T>
+__host__ __device__ constexpr bool c2r2h_supported(int N) {
+    return c2r2d_supported<T>(N) && N % 4 == 0 && reg_fft_a(N / 2) > 0 && N >= (sizeof(T) == 4 ? 144 : 96);
+}
+
+
 template <typename T, int N, int NT>
 __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
     constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1;
@@ -197,6 +211,153 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
             }
         }
         // the next round touches other rows only: no barrier
+    }
+}
+
+// Half-plane c2r: the block of job (plane, r) produces the physical rows
+// 2m + r of the plane (r = 0, 1).  With n2 = 2m + r the inverse x2 transform
+// of a column is x[2m+r] = Σ_{k<M} e^{2πi·mk/M}·Y_r[k], M = N/2, with
+// Y_r[k] = (X[k] + (−1)^r·X[k+M])·e^{2πi·rk/N}, so the column FFTs have length
+// M and the intermediate holds M rows; the x1 pass is the whole-plane one on
+// those rows.  The plane's inputs are read by both blocks of the plane.
+template <typename T, int N, int NT>
+__device__ __forceinline__ void c2r_2h_reg(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
+    constexpr int M = N / 2;
+    constexpr int AM = reg_fft_a(M), BM = M / AM;  // column FFT
+    constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1;  // row FFT
+    constexpr int NP = M / 2;  // row pairs of the half plane
+    constexpr int ROUNDS = (NP * Bq + NT - 1) / NT;
+    constexpr int P = (NP + ROUNDS - 1) / ROUNDS;
+    static_assert(P * Bq <= NT, "one first-stage row item per thread");
+    const int jb = blockIdx.x - E.block_begin;
+    const int job = jb >> 1, r = jb & 1;
+    int term = 0;
+    while (job >= S.in_job0[term + 1]) ++term;
+    const int layer = job - S.in_job0[term];
+    const int recipe = S.in_recipe[term];
+    const int nk1 = S.nk1, zs = c2r2h_zs(N, nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    cplx<T> *Z = smem;
+    cplx<T> *TW = smem + size_t(M) * zs;
+    const size_t P2 = size_t(K1) * K2;
+    for (int j = threadIdx.x; j < N; j += NT) TW[j] = gload(S.tw + S.p2.tw + j);
+    const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
+    const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;  // w_N^j; w_M^j = w_N^{2j}
+
+    // x2, stage 1: item (c, t); AM values Y_r[k] at k = BM·n + t
+    for (int it = threadIdx.x; it < nk1 * BM; it += NT) {
+        const int t = it / nk1, c = it - t * nk1;
+        const size_t base = size_t(layer) * P2 + c;
+        cplx<T> v[AM];
+        const T sg = r ? T(-1) : T(1);
+        // resolved index of k2 position p (clamped to 0 where p is padding)
+        auto idx = [&](int p, bool &ok) {
+            const bool lo = p <= k2m, hi = p >= N - k2m;
+            ok = lo || hi;
+            return lo ? p : (hi ? K2 - (N - p) : 0);
+        };
+        auto gather = [&](auto value) {
+            static_for<0, AM>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                bool oka, okb;
+                const int ia = idx(BM * n + t, oka), ib = idx(BM * n + t + M, okb);
+                const cplx<T> xa = value(base + size_t(K1) * ia, ia);
+                const cplx<T> xb = value(base + size_t(K1) * ib, ib);
+                const cplx<T> a = oka ? xa : czero<T>(), b = okb ? xb : czero<T>();
+                v[n] = mk(a.x + sg * b.x, a.y + sg * b.y);
+            });
+        };
+        const T d1 = T(gload(S.D1 + c));
+        switch (recipe) {  // uniform per block
+            case R_U: gather([&](size_t i, int) { return gload(u + i); }); break;
+            case R_V: gather([&](size_t i, int) { return gload(vv + i); }); break;
+            case R_W: gather([&](size_t i, int) { return gload(w + i); }); break;
+            case R_D1U: gather([&](size_t i, int) { return imul(d1, gload(u + i)); }); break;
+            case R_D1V: gather([&](size_t i, int) { return imul(d1, gload(vv + i)); }); break;
+            case R_D1W: gather([&](size_t i, int) { return imul(d1, gload(w + i)); }); break;
+            case R_D2U: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(u + i)); }); break;
+            case R_D2V: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(vv + i)); }); break;
+            case R_D2W: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(w + i)); }); break;
+            default: gather([&](size_t i, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
+        }
+        if (r) {  // e^{2πi·k/N}
+            static_for<1, AM>([&](auto nc) {
+                constexpr int n = decltype(nc)::value;
+                v[n] = cmul(v[n], gtwiddle<T, true>(tw, BM * n + t));
+            });
+            if (t) v[0] = cmul(v[0], gtwiddle<T, true>(tw, t));
+        }
+        rdft<T, true, AM>(v);
+        static_for<0, AM>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            Z[(t * AM + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw, 2 * ((t * k) % M)));
+        });
+    }
+    __syncthreads();
+    // x2, stage 2 in place: local rows t + AM·k
+    for (int it = threadIdx.x; it < nk1 * AM; it += NT) {
+        const int t = it / nk1, c = it - t * nk1;
+        cplx<T> y[BM];
+        static_for<0, BM>([&](auto nc) {
+            constexpr int n = decltype(nc)::value;
+            y[n] = Z[(n * AM + t) * zs + c];
+        });
+        rdft<T, true, BM>(y);
+        static_for<0, BM>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            cplx<T> val = y[k];
+            if (c == 0) val.y = T(0);  // c2r uses the real part of the k1 = 0 column
+            Z[(t + AM * k) * zs + c] = val;
+        });
+    }
+    __syncthreads();
+    // x1 on the M local rows (physical rows 2m + r), pairs of local rows in place
+    T *out = S.in_phys[term] + size_t(layer) * N * N;
+    for (int p0 = 0; p0 < NP; p0 += P) {
+        const int it = threadIdx.x;
+        const int pl = it / Bq, t = it - pl * Bq;
+        const bool own = pl < P && p0 + pl < NP;
+        const int p = min(p0 + pl, NP - 1);
+        cplx<T> *R = Z + size_t(2 * p) * zs;
+        cplx<T> v[A];
+        static_for<0, A>([&](auto nc) {
+            constexpr int n = decltype(nc)::value;
+            const int k = Bq * n + t;
+            const bool direct = k <= k1m, mirror = !direct && N - k <= k1m;
+            const bool real_only = k == 0 || (2 * k == N && direct);
+            const int kk = direct ? k : (mirror ? N - k : 0);
+            const cplx<T> a = R[kk], b = R[zs + kk];
+            cplx<T> z = real_only ? mk(a.x, b.x) : direct ? mk(a.x - b.y, a.y + b.x) : mk(a.x + b.y, b.x - a.y);
+            if (!(direct || mirror)) z = czero<T>();
+            v[n] = z;
+        });
+        __syncthreads();
+        if (own) {
+            rdft<T, true, A>(v);
+            static_for<0, A>([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                R[t * RS + k] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, t * k));
+            });
+        }
+        __syncthreads();
+        for (int i2 = threadIdx.x; i2 < P * A; i2 += NT) {
+            const int ql = i2 / A, s = i2 - ql * A;
+            const int q = p0 + ql;
+            if (q < NP) {
+                const cplx<T> *Q = Z + size_t(2 * q) * zs;
+                cplx<T> y[Bq];
+                static_for<0, Bq>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    y[n] = Q[n * RS + s];
+                });
+                rdft<T, true, Bq>(y);
+                T *ra = out + size_t(2 * (2 * q) + r) * N, *rb = out + size_t(2 * (2 * q + 1) + r) * N;
+                static_for<0, Bq>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    gstore(ra + s + A * k, y[k].x);
+                    gstore(rb + s + A * k, y[k].y);
+                });
+            }
+        }
     }
 }
 

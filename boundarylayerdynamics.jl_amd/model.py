@@ -689,7 +689,7 @@ class Model:
         self._check(self._lib.bld_tuning(self._h, 4, 0, ctypes.byref(b), ctypes.byref(us)))
         out["stage_chunk"] = (b.value, round(us.value, 2))
         self._check(self._lib.bld_tuning(self._h, 5, 0, ctypes.byref(b), ctypes.byref(us)))
-        out["transform_path"] = ({"c2r": "whole-plane" if b.value & 1 else "two-pass",
+        out["transform_path"] = ({"c2r": ("half-plane" if b.value & 4 else "whole-plane") if b.value & 1 else "two-pass",
                                   "r2c": "whole-plane" if b.value & 2 else "two-pass"}, round(us.value, 2))
         return out
 

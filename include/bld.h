@@ -126,7 +126,8 @@ int bld_autotune(bld_engine *e, int reps);
  * 2 r2c_x1, 3 r2c_x2) for the entry-th model space; pass 4: the chunk length
  * of the fused stage kernel and the time of one stage; pass 5: the transform
  * path (bit 0: whole-plane c2r, bit 1: whole-plane r2c, for the square grids
- * that fit in LDS) and the time of one right-hand side (entry ignored). */
+ * that fit in LDS; bit 2: c2r of the largest planes by half planes) and the
+ * time of one right-hand side (entry ignored). */
 int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us);
 
 /* Learned SGS closure (no reference counterpart; it replaces the stresses of

@@ -60,6 +60,16 @@ CASES = {
                        upper="FreeSlipBoundary",
                        procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
                                                              sgs_model=M.StaticSmagorinskyModel(Cs=0.1))),
+    # larger square LES grids: advection 96x96 (fp64: half-plane c2r blocks) and
+    # 144x144 (fp32: half-plane c2r blocks)
+    "les_64": dict(res=(64, 64, 6), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4),
+                   upper="FreeSlipBoundary",
+                   procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
+                                                         sgs_model=M.StaticSmagorinskyModel(Cs=0.1))),
+    "les_96": dict(res=(96, 96, 5), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4),
+                   upper="FreeSlipBoundary",
+                   procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
+                                                         sgs_model=M.StaticSmagorinskyModel(Cs=0.1))),
     # closed channel LES with two rough walls, no damping (les_test.jl:1-48)
     "les_closed": dict(res=(12, 14, 6), dims=(2 * np.pi, 4 / 3 * np.pi, 2.0), lower=("RoughWall", 5e-3),
                        upper=("RoughWall", 5e-3),
@@ -178,7 +188,7 @@ def test_physical_transfer(name):
         assert relerr(gpu.state[f], exp) < 1e-13
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom", "les_square"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "les_square", "les_64", "les_96"])
 def test_rates_fp32_mixed(name):
     ref, gpu = pair(name, precision="fp32", seed=9)
     rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
@@ -389,7 +399,7 @@ def test_ensemble_fp32_matches_single_models():
             assert relerr(a[f], b[f]) < 1e-5, (f, relerr(a[f], b[f]))
 
 
-@pytest.mark.parametrize("name", ["les", "channel", "les_square"])
+@pytest.mark.parametrize("name", ["les", "channel", "les_square", "les_64"])
 def test_autotune_keeps_results(name):
     ref, gpu = pair(name, seed=60)
     before = gpu.rates()
