@@ -30,7 +30,7 @@ KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "
 EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_size", "bld_select_instance",
            "bld_autotune", "bld_tuning", "bld_last_error", "bld_set_state", "bld_get_state", "bld_rates",
            "bld_project", "bld_init_problem", "bld_step", "bld_step_graph", "bld_set_field_physical",
-           "bld_get_field_physical", "bld_time_steps", "bld_set_profiling", "bld_kernel_times",
+           "bld_get_field_physical", "bld_time_steps", "bld_set_profiling", "bld_kernel_times", "bld_time_kernel",
            "bld_ldlt_create", "bld_ldlt_destroy", "bld_ldlt_solve", "bld_ldlt_bench", "bld_ldlt_last_error",
            "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
            "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time",
@@ -99,6 +99,7 @@ def load():
         "bld_time_steps": ([P, ctypes.c_double, i64, ctypes.c_int, D], ctypes.c_int),
         "bld_set_profiling": ([P, ctypes.c_int], ctypes.c_int),
         "bld_kernel_times": ([P, ctypes.c_int, D, ctypes.POINTER(i64)], ctypes.c_int),
+        "bld_time_kernel": ([P, ctypes.c_int, ctypes.c_int, D, ctypes.POINTER(i64)], ctypes.c_int),
         "bld_ldlt_create": ([i64, i64, D, D, ctypes.c_int, ctypes.POINTER(P)], ctypes.c_int),
         "bld_ldlt_destroy": ([P], None),
         "bld_ldlt_solve": ([P, D], ctypes.c_int),

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <new>
@@ -149,6 +150,7 @@ struct EngineBase {
     virtual double time_steps(double dt, int64_t n, bool graph) = 0;
     virtual void set_profiling(bool on) = 0;
     virtual void kernel_time(int kind, double *ms, int64_t *count) = 0;
+    virtual void replay_kernel(int kind, int reps, double *us, int64_t *dispatches) = 0;
     virtual void autotune(int reps) = 0;
     virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
     virtual void set_closure(const float *params, double delta, const double *xw, int flags) = 0;
@@ -1321,6 +1323,74 @@ class Engine : public EngineBase {
         *count = kt_n[kind];
     }
 
+    // The dispatches of one kernel class in a right-hand side of state u, replayed
+    // reps times back to back between two events on the engine stream: the
+    // average duration per dispatch without the dispatch latency that an event
+    // pair around every single dispatch adds.  The rates are recomputed after.
+    void replay_kernel(int kind, int reps, double *us, int64_t *dispatches) override {
+        if (reps < 1) throw BldError{BLD_EARG, "reps must be positive"};
+        const Space<T> *dsp = d_spaces.as<Space<T>>();
+        std::vector<std::pair<int, const Pass *>> tr;  // transform launches (kind, pass)
+        auto add = [&](int k, const std::vector<Pass> &v) {
+            for (const Pass &p : v)
+                if (p.nblocks) tr.push_back({k, &p});
+        };
+        int n_other = 0;
+        std::function<void()> other;
+        switch (kind) {
+            case KK_C2R_2D: {
+                const int kinds[3] = {PASS_C2R_2D, PASS_C2R_2D_HALF, PASS_C2R_2H};
+                for (auto &g : tp_c2r2d)
+                    for (int h = 0; h < 3; ++h) add(kinds[h], g[h]);
+                break;
+            }
+            case KK_R2C_2D: add(PASS_R2C_2D, tp_r2c_2d); add(PASS_R2C_2D, tp_r2c_2d_sgs); break;
+            case KK_C2R_X2: add(PASS_C2R_X2, tp[PASS_C2R_X2]); break;
+            case KK_C2R_X1: add(PASS_C2R_X1, tp[PASS_C2R_X1]); break;
+            case KK_R2C_X1: add(PASS_R2C_X1, tp[PASS_R2C_X1]); break;
+            case KK_R2C_X2: add(PASS_R2C_X2, tp[PASS_R2C_X2]); add(PASS_R2C_X2, tp_r2c_x2_sgs); break;
+            case KK_LINEAR:
+                n_other = 1;
+                other = [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, 0, st); };
+                break;
+            case KK_ADV:
+            case KK_SGS_VISC:
+            case KK_SGS_STRESS: {
+                const int pk = kind == KK_ADV ? PHYS_ADV : (kind == KK_SGS_VISC ? PHYS_SGS_VISC : PHYS_SGS_STRESS);
+                if (pp[pk].nblocks) {
+                    n_other = 1;
+                    other = [&, pk] { launch_physics<T>(pk, dsp, pp[pk].lt(), pp[pk].nblocks, st); };
+                }
+                break;
+            }
+            default: throw BldError{BLD_EUNSUPPORTED, "kernel class cannot be replayed on its own"};
+        }
+        const int64_t n = int64_t(tr.size()) + n_other;
+        *dispatches = n;
+        *us = 0.0;
+        if (n == 0) return;
+        rhs(0);  // every input of the class holds live data
+        auto once = [&] {
+            for (auto &e : tr) launch_transform<T>(e.first, dsp, e.second->lt(), e.second->nblocks, e.second->smem, 0, st);
+            if (other) other();
+        };
+        once();
+        hipEvent_t a, b;
+        HIPCHECK(hipEventCreate(&a));
+        HIPCHECK(hipEventCreate(&b));
+        HIPCHECK(hipEventRecord(a, st));
+        for (int r = 0; r < reps; ++r) once();
+        HIPCHECK(hipEventRecord(b, st));
+        HIPCHECK(hipEventSynchronize(b));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, a, b));
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *us = 1e3 * ms / double(reps * n);
+        rhs(0);  // the r2c classes accumulate into du: recompute the rates
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
     // -- FFT batch autotuner ------------------------------------------------------
     // For every transform pass and model space, times each power-of-two batch
     // (columns or row pairs per block) whose LDS fits, on the live data, and keeps
@@ -2123,6 +2193,14 @@ int bld_kernel_times(bld_engine *e, int kind, double *ms_total, int64_t *launche
     ENGINE_CALL({
         if (!ms_total || !launches) throw BldError{BLD_EARG, "null output"};
         e->impl->kernel_time(kind, ms_total, launches);
+        return BLD_OK;
+    })
+}
+
+int bld_time_kernel(bld_engine *e, int kind, int reps, double *us_per_dispatch, int64_t *dispatches) {
+    ENGINE_CALL({
+        if (!us_per_dispatch || !dispatches) throw BldError{BLD_EARG, "null output"};
+        e->impl->replay_kernel(kind, reps, us_per_dispatch, dispatches);
         return BLD_OK;
     })
 }

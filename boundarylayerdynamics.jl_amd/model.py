@@ -715,6 +715,14 @@ class Model:
                                                  ctypes.byref(n)))
         return x, t, delta
 
+    def time_kernel(self, name, reps=20):
+        """(µs per dispatch, dispatches per right-hand side) of kernel class
+        `name` replayed back to back (bld_time_kernel)."""
+        us, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(self._lib.bld_time_kernel(self._h, _lib.KERNEL_KINDS.index(name), reps, ctypes.byref(us),
+                                              ctypes.byref(n)))
+        return us.value, n.value
+
     def kernel_times(self):
         out = {}
         for k, name in enumerate(_lib.KERNEL_KINDS):

@@ -219,6 +219,13 @@ int bld_time_steps(bld_engine *e, double dt, int64_t nsteps, int use_graph, doub
  * bld_time_steps call when profiling is enabled (bld_set_profiling). */
 int bld_set_profiling(bld_engine *e, int enabled);
 int bld_kernel_times(bld_engine *e, int kind, double *ms_total, int64_t *launches);
+/* Measurement hook (no counterpart in the reference): the dispatches of one
+ * kernel class (same kind numbering) of a right-hand side replayed reps times
+ * back to back between two HIP events on the engine stream; returns the
+ * average duration of one dispatch and the dispatches per right-hand side.
+ * The transform, linear-term and pointwise physics classes can be replayed;
+ * the engine's rates are recomputed afterwards. */
+int bld_time_kernel(bld_engine *e, int kind, int reps, double *us_per_dispatch, int64_t *dispatches);
 
 /* Batched symmetric tridiagonal LDLt (DistributedBatchLDLt + ldiv!,
  * src/processes/pressure.jl:38-201): nk systems of size nz.  dvs [nz][nk],

@@ -416,6 +416,23 @@ def test_autotune_keeps_results(name):
         assert relerr(got[f], ref.state[f]) < RTOL64
 
 
+@pytest.mark.parametrize("name", ["les_square", "channel"])
+def test_kernel_replay_keeps_rates(name):
+    """bld_time_kernel replays a kernel class on its own (measurement hook):
+    positive durations, the dispatch count of one right-hand side, and the
+    rates unchanged afterwards (the accumulating r2c classes are recomputed)."""
+    ref, gpu = pair(name, precision="fp32", seed=61)
+    before = gpu.rates()
+    for k in ("c2r_2d", "r2c_2d", "c2r_x2", "r2c_x2", "linear", "adv_products", "sgs_viscosity", "sgs_stress"):
+        us, n = gpu.time_kernel(k, reps=3)
+        assert n >= 0 and (us > 0) == (n > 0), (k, us, n)
+    with pytest.raises(B._lib.BldError):
+        gpu.time_kernel("stage_project")
+    after = gpu.rates()
+    for f in ref.fields:
+        assert relerr(after[f], before[f]) < 1e-6
+
+
 @pytest.mark.parametrize("method", ["Euler", "SSPRK22", "AB2"])
 @pytest.mark.parametrize("name,dt", [("channel", 1e-3), ("les", 5e-4), ("channel_flux", 1e-3)])
 @pytest.mark.parametrize("graph", [False, True])
