@@ -1028,20 +1028,14 @@ class Engine : public EngineBase {
         run_transform(PASS_R2C_2D, tp_r2c_2d_sgs, sel, KK_R2C_2D);
     }
 
-    // The float64 (0,0) pass of the mixed form touches only the (0,0) columns,
-    // which the main pass skips: it runs concurrently on the side stream.
+    // The float64 (0,0) column of each mixed-form model is taken by extra blocks
+    // of the same launch (the float pass skips it), so a stage is one kernel
+    // on the engine stream.
     void stage(int s, double dt) {
         timed(KK_STAGE, [&] {
-            if (stage00_pass.nblocks) {
-                HIPCHECK(hipEventRecord(ev_fork, st));
-                HIPCHECK(hipStreamWaitEvent(st2, ev_fork, 0));
-                launch_stage_chunked<double>(stage_CL, d_proj00.as<ProjDesc<double>>(), stage00_pass.lt(),
-                                             stage00_pass.nblocks, s, dt, st2);
-                HIPCHECK(hipEventRecord(ev_join, st2));
-            }
-            launch_stage_chunked<T>(stage_CL, d_proj.as<ProjDesc<T>>(), stage_pass.lt(), stage_pass.nblocks, s, dt,
-                                    st);
-            if (stage00_pass.nblocks) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
+            launch_stage_chunked<T>(stage_CL, d_proj.as<ProjDesc<T>>(), stage_pass.lt(), stage_pass.nblocks, s, dt, st,
+                                    stage00_pass.nblocks ? d_proj00.as<ProjDesc<double>>() : nullptr,
+                                    stage00_pass.nblocks);
             for (auto &I : inst)
                 if (I->has_mean) launch_constant_mean<T>(I->cmean, s, st);
             for (auto &I : inst)

@@ -820,18 +820,15 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
 // In the first (forward) / last (backward) chunk this is the sequential
 // recurrence of pressure.jl:126-160 operation for operation.
 template <typename T, int CL>
-__global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__restrict__ descs, LaunchTable lt, int stage,
-                                                       double dt) {
-    extern __shared__ unsigned char smem_raw[];
-    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
-    const ProjDesc<T> &D = descs[E.space];
+__device__ __forceinline__ void stage_chunked_body(const ProjDesc<T> &D, int blk, int stage, double dt,
+                                                   unsigned char *smem_raw) {
     const int MB = D.MB, skip0 = D.skip0;
     const int NC = blockDim.x / MB;
     cplx<double> *s_y = reinterpret_cast<cplx<double> *>(smem_raw);  // [NC][MB]
     double *s_p = reinterpret_cast<double *>(s_y + NC * MB);         // [NC][MB]
     cplx<T> *s_w = reinterpret_cast<cplx<T> *>(s_p + NC * MB);      // [NC][MB]
     const int tx = threadIdx.x % MB, ty = threadIdx.x / MB;
-    const int m = (blockIdx.x - E.block_begin) * MB + tx;
+    const int m = blk * MB + tx;
     const int nk = D.nk, n3 = D.n3;
     const bool active = m < nk && !(skip0 && m == 0);
     const int l0 = ty * CL;
@@ -1050,6 +1047,22 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
     }
 }
 
+// One launch of the stage kernel: blocks below nmain cover the columns of
+// every model (launch table); the blocks after them take the float64 (0,0)
+// column of each mixed-form model, so that it needs no second stream.
+template <typename T, int CL>
+__global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__restrict__ descs, LaunchTable lt, int stage,
+                                                       double dt, const ProjDesc<double> *__restrict__ descs00,
+                                                       int nmain) {
+    extern __shared__ unsigned char smem_raw[];
+    if (int(blockIdx.x) >= nmain) {
+        if constexpr (sizeof(T) != sizeof(double)) stage_chunked_body<double, CL>(descs00[blockIdx.x - nmain], 0, stage, dt, smem_raw);
+        return;
+    }
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    stage_chunked_body<T, CL>(descs[E.space], blockIdx.x - E.block_begin, stage, dt, smem_raw);
+}
+
 // ConstantMean projection (sources.jl:63-77): shift the (0,0) column of each
 // field with a prescribed mean so that Σ_l w_l·Re q(0,0,l) equals it; one block
 // per instance, the float64 shadow first where the mixed form keeps one
@@ -1156,17 +1169,18 @@ void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nbloc
 // CL is the chunk length shared by the launch (block = 256 threads = MB·NC).
 template <typename T>
 void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int stage,
-                          double dt, hipStream_t st) {
-    if (nblocks <= 0) return;
-    const size_t smem = 256 * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<T>));
+                          double dt, hipStream_t st, const ProjDesc<double> *descs00_dev, int n00) {
+    if (nblocks + n00 <= 0) return;
+    const int grid = nblocks + n00;
+    const size_t smem = 256 * (sizeof(cplx<double>) + sizeof(double) + sizeof(cplx<double>));
     if (CL == 2)
-        k_stage_chunked<T, 2><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
+        k_stage_chunked<T, 2><<<grid, 256, smem, st>>>(descs_dev, lt, stage, dt, descs00_dev, nblocks);
     else if (CL == 4)
-        k_stage_chunked<T, 4><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
+        k_stage_chunked<T, 4><<<grid, 256, smem, st>>>(descs_dev, lt, stage, dt, descs00_dev, nblocks);
     else if (CL == 8)
-        k_stage_chunked<T, 8><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
+        k_stage_chunked<T, 8><<<grid, 256, smem, st>>>(descs_dev, lt, stage, dt, descs00_dev, nblocks);
     else
-        k_stage_chunked<T, 16><<<nblocks, 256, smem, st>>>(descs_dev, lt, stage, dt);
+        k_stage_chunked<T, 16><<<grid, 256, smem, st>>>(descs_dev, lt, stage, dt, descs00_dev, nblocks);
 }
 
 // one thread per column (very tall columns, n3 > 16·16)
@@ -1192,7 +1206,8 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
     template void launch_physics<T>(int, const Space<T> *, const LaunchTable &, int, hipStream_t);                \
     template void launch_linear<T>(const LinDesc<T> *, const LaunchTable &, int, int, hipStream_t);               \
     template void launch_constant_mean<T>(const CMeanDesc<T> &, int, hipStream_t);                                \
-    template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t); \
+    template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t, \
+                                          const ProjDesc<double> *, int);                                        \
     template void launch_stage_sequential<T>(const ProjDesc<T> &, int, double, hipStream_t);                      \
     template void launch_check_finite<T>(const void *, size_t, int *, hipStream_t);                               \
     template void launch_fill<T>(void *, size_t, hipStream_t);                                                    \

@@ -124,7 +124,7 @@ template <typename T>
 void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st);
 template <typename T>
 void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int stage,
-                          double dt, hipStream_t st);
+                          double dt, hipStream_t st, const ProjDesc<double> *descs00_dev = nullptr, int n00 = 0);
 template <typename T>
 void launch_stage_sequential(const ProjDesc<T> &D, int stage, double dt, hipStream_t st);
 template <typename T>
