@@ -358,8 +358,9 @@ def rocprof_summary(kt_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir
     byp = {}
     for r in prof:
         byp.setdefault(_short(r["Kernel_Name"]), []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
-    lines += ["", "Profiled steps (serialized on one stream, after bench.py's idle gap; the dispatches "
-                  "bench.py's per-kernel averages and roofline come from):", "",
+    lines += ["", "Profiled steps (serialized on one stream, after bench.py's idle gap: the profiled steps "
+                  "behind bench.py's per-kernel averages, then its back-to-back replay of the dominant class "
+                  "for the roofline):", "",
               "| kernel | dispatches | average us |", "|---|---|---|"]
     for k, d in sorted(byp.items(), key=lambda kv: -sum(kv[1])):
         lines.append(f"| {k} | {len(d)} | {sum(d) / len(d) / 1e3:.2f} |")
@@ -391,10 +392,11 @@ def measured_traffic(kernel):
     one kernel per block size), or None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
+    names = {"c2r_2d": ("k_c2r_2d<float", "k_c2r_2h<float")}.get(kernel, ("k_" + kernel + "<float",))
     tot, n = 0.0, 0
     for line in open(TRAFFIC_FILE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith("k_" + kernel + "<float"):
+        if len(cells) == 5 and cells[0].startswith(names):
             try:
                 tot += float(cells[4]) * 1e6 * int(cells[1])
                 n += int(cells[1])

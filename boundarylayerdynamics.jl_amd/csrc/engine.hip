@@ -1491,19 +1491,17 @@ class Engine : public EngineBase {
             bool any_half = false;
             for (auto &g : tp_c2r2d) any_half |= !g[2].empty();
             if (any_half) {
-                double bh_us = 1e30;
-                int bh = 1;
+                // whole-plane blocks stay unless half planes are clearly faster
+                // (≥ 3 %), so that the choice does not follow timing noise
+                double us_h[2];
                 for (int h = 1; h >= 0; --h) {
                     half_c2r = h == 1;
                     build_transform_passes(&best);
-                    const double us = time_us([&] { rhs(0); }, reps);
-                    if (us < bh_us) {
-                        bh_us = us;
-                        bh = h;
-                    }
+                    us_h[h] = time_us([&] { rhs(0); }, reps);
                 }
+                const int bh = us_h[1] < 0.97 * us_h[0] ? 1 : 0;
                 half_c2r = bh == 1;
-                bt = bh_us;
+                bt = us_h[bh];
                 bm = (bm & 3) | (bh << 2);
                 build_transform_passes(&best);
             }
