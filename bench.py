@@ -385,21 +385,46 @@ def rocprof_summary(kt_dir, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_les128x128x64_fp32.md")
 
 
+KERNELS_FILE = os.path.join(REPO, "profiles", "les128x128x64_fp32_kernels.md")
+
+
+def _md_rows(path, ncells):
+    """The table rows with ncells cells of a markdown file."""
+    out = []
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if line.startswith("|") and len(cells) == ncells:
+            out.append(cells)
+    return out
+
+
 def measured_traffic(kernel):
     """HBM bytes per dispatch of kernel class `kernel` (float32 kernels) from the
     committed counter summary (the 'traffic MB (scaled)' column): the mean over
-    the class's kernels weighted by their dispatches (a class such as c2r_2d has
-    one kernel per block size), or None."""
-    if not os.path.exists(TRAFFIC_FILE):
+    the class's kernels (a class such as c2r_2d has one kernel per block size)
+    weighted by their dispatches in the profiled steps of the committed kernel
+    summary, so that kernels only the autotuner tried do not count; or None."""
+    if not (os.path.exists(TRAFFIC_FILE) and os.path.exists(KERNELS_FILE)):
         return None
     names = {"c2r_2d": ("k_c2r_2d<float", "k_c2r_2h<float")}.get(kernel, ("k_" + kernel + "<float",))
-    tot, n = 0.0, 0
-    for line in open(TRAFFIC_FILE):
+    text = open(KERNELS_FILE).read()
+    if "Profiled steps" not in text:
+        return None
+    prof = text[text.index("Profiled steps"):]
+    weight = {}
+    for line in prof.splitlines():
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0].startswith(names):
+        if line.startswith("|") and len(cells) == 3 and cells[0].startswith(names):
             try:
-                tot += float(cells[4]) * 1e6 * int(cells[1])
-                n += int(cells[1])
+                weight[cells[0]] = int(cells[1])
+            except ValueError:
+                pass
+    tot, n = 0.0, 0
+    for cells in _md_rows(TRAFFIC_FILE, 5):
+        if cells[0] in weight:
+            try:
+                tot += float(cells[4]) * 1e6 * weight[cells[0]]
+                n += weight[cells[0]]
             except ValueError:
                 return None
     return int(tot / n) if n else None
