@@ -94,3 +94,23 @@ def test_unsupported_processes_rejected_on_host():
         B.model.Discretization((4, 4, 4), dom, [B.MolecularDiffusion("theta", 1.0)])
     with pytest.raises(ValueError):
         B.model.Discretization((4, 4, 2), dom, [B.Pressure()])
+
+
+def test_whole_plane_rule_matches_the_lds_layouts():
+    """The byte model's replica of the engine's whole-plane eligibility rule:
+    the 128x128x64 LES grids (192² advection, 128² SGS) take both whole-plane
+    transforms in float32; float64 takes them up to 96² only; non-square and
+    odd grids never.  The LDS sizes follow the layouts of transforms_2d.hpp:
+    Z rows of max(nk1, ⌈B(A+1)/2⌉) entries spread to ≡ B/2 (mod 16), plus the
+    N twiddles (c2r); Z rows of nk1 entries spread the same way, one round of
+    the row-pair exchange, the twiddles and the D1 factors (r2c)."""
+    W = load_package().workloads
+    # c2r, 192² fp32: A = 16, B = 12, zs = 102 -> (192*102 + 192) * 8 B = 158,208 B
+    assert W.c2r_fused(192, 192, 64, "fp32") and (192 * 102 + 192) * 8 == 158208
+    # r2c, 192² fp32: zs = 70, 32 pairs of 12*17 entries -> 161,792 B <= 160 KiB
+    assert W.r2c_fused(192, 192, 64, "fp32") and (192 * 70 + 32 * 12 * 17 + 192 + 64) * 8 == 161792
+    assert W.c2r_fused(128, 128, 64, "fp32") and W.r2c_fused(128, 128, 64, "fp32")
+    assert not W.c2r_fused(192, 192, 64, "fp64") and not W.r2c_fused(192, 192, 64, "fp64")
+    assert W.c2r_fused(96, 96, 33, "fp64")
+    assert not W.c2r_fused(18, 15, 7, "fp32") and not W.c2r_fused(30, 30, 11, "fp32")  # non-square / no register FFT
+    assert not W.c2r_fused(384, 384, 128, "fp32")  # plane beyond LDS
