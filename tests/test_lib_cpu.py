@@ -114,3 +114,39 @@ def test_whole_plane_rule_matches_the_lds_layouts():
     assert W.c2r_fused(96, 96, 33, "fp64")
     assert not W.c2r_fused(18, 15, 7, "fp32") and not W.c2r_fused(30, 30, 11, "fp32")  # non-square / no register FFT
     assert not W.c2r_fused(384, 384, 128, "fp32")  # plane beyond LDS
+
+
+def _ldlt_chunked_max_index(nk, nz, refine, CL=16):
+    """Largest element index of the global loads of the chunked LDLt kernel
+    (k_ldlt_chunked, 256-thread blocks of MB systems × NC chunks of CL levels):
+    the address expressions of its loads, levels clamped into the column."""
+    NC = 1
+    while NC * CL < nz:
+        NC *= 2
+    MB = 256 // NC
+    worst = 0
+    for t in range(256):
+        ty = t // MB
+        for blk in range((nk + MB - 1) // MB):
+            m = blk * MB + t % MB
+            mm = m if m < nk else 0
+            for j in range(CL):
+                l = ty * CL + j
+                lc = min(l, nz - 1)
+                idx = [lc * nk + mm, max(lc - 1, 0) * nk + mm, min(lc, max(nz - 2, 0)) * nk + mm]
+                if refine:
+                    if l > 0:
+                        idx.append(min(max(l - 1, 0), max(nz - 2, 0)) * nk + mm)
+                    if l + 1 < nz:
+                        idx.append(min(l, max(nz - 2, 0)) * nk + mm)
+                worst = max(worst, *idx)
+    return worst
+
+
+@pytest.mark.parametrize("nk,nz", [(6, 3), (6, 16), (37, 17), (37, 300), (1024, 64)])
+def test_ldlt_chunked_loads_stay_in_bounds(nk, nz):
+    """The padding levels of the last chunk (levels nz..NC·CL−1) load at
+    clamped levels; the float32 refinement's β below a level once read past
+    the end of β for such levels (up to (NC·CL−nz)·nk elements)."""
+    for refine in (False, True):
+        assert _ldlt_chunked_max_index(nk, nz, refine) < nk * nz
