@@ -1,5 +1,6 @@
 // Host side of the time-step engine: C ABI (include/bld.h), device memory,
 // FFT plans, launch sequencing and HIP-graph capture of SSPRK33 steps.
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -144,14 +145,17 @@ struct EngineBase {
     virtual int rates(double *const f[3]) = 0;
     virtual void project() = 0;
     virtual int init_problem() = 0;
-    virtual int step(double dt, int64_t n, bool graph) = 0;
+    // returns BLD_OK or BLD_EDIVERGED; *done: time advances made (all n, or
+    // the steps before the reference would have raised)
+    virtual int step(double dt, int64_t n, bool graph, int64_t *done) = 0;
     virtual void set_field_physical(int f, const double *v, bool centered) = 0;
     virtual void get_field_physical(int f, double *v, bool centered) = 0;
-    virtual double time_steps(double dt, int64_t n, bool graph) = 0;
+    virtual double time_steps(double dt, int64_t n, bool graph, bool *diverged) = 0;
     virtual void set_profiling(bool on) = 0;
     virtual void kernel_time(int kind, double *ms, int64_t *count) = 0;
     virtual void replay_kernel(int kind, int reps, double *us, int64_t *dispatches) = 0;
     virtual void autotune(int reps) = 0;
+    virtual void set_transform_path(int path) = 0;
     virtual void tuning(int pass, int entry, int *batch, double *us) = 0;
     virtual void set_closure(const float *params, double delta, const double *xw, int flags) = 0;
     virtual void set_method(int method) = 0;
@@ -245,12 +249,14 @@ class Engine : public EngineBase {
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         HIPCHECK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
         HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_fork2, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_lin, hipEventDisableTiming));
-        d_flag.alloc(sizeof(int));
-        HIPCHECK(hipMemset(d_flag.p, 0, sizeof(int)));
+        for (auto &ev : ev_chunk) HIPCHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHECK(hipHostMalloc(reinterpret_cast<void **>(&h_flag), 2 * 2 * sizeof(int), hipHostMallocDefault));
+        d_flag.alloc(2 * sizeof(int));
+        clear_flag();
+        HIPCHECK(hipStreamSynchronize(st));
         for (int i = 0; i < n; ++i) {
             inst.emplace_back(new Instance<T>());
             build_instance(*inst.back(), cfgs[i]);
@@ -266,8 +272,10 @@ class Engine : public EngineBase {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         for (auto &ev : events) (void)hipEventDestroy(ev);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
         if (ev_fork2) (void)hipEventDestroy(ev_fork2);
+        for (auto &ev : ev_chunk)
+            if (ev) (void)hipEventDestroy(ev);
+        if (h_flag) (void)hipHostFree(h_flag);
         if (ev_join2) (void)hipEventDestroy(ev_join2);
         if (ev_lin) (void)hipEventDestroy(ev_lin);
         if (st2) (void)hipStreamDestroy(st2);
@@ -275,10 +283,14 @@ class Engine : public EngineBase {
     }
 
     hipStream_t st = nullptr;
-    // side stream of the float64 (0,0)-mode stage pass, forked from and joined
-    // back into `st` (also inside the captured graph)
+    // side stream of the advection chain of a right-hand side, forked from and
+    // joined back into `st` (also inside the captured graph)
     hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr, ev_lin = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_fork2 = nullptr, ev_join2 = nullptr, ev_lin = nullptr;
+    // divergence flag: d_flag[0] first diverged step count (INT_MAX: none),
+    // d_flag[1] steps counted (record_diverged); pinned copies per step chunk
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+    int *h_flag = nullptr;
     std::vector<std::unique_ptr<Instance<T>>> inst;
     int sel_inst = 0;
     TwiddleSet<T> tw;
@@ -315,6 +327,7 @@ class Engine : public EngineBase {
     double ab2_dt = 0;  // chunk length chosen by the autotuner (0: by n3)
     // tuning results: per transform pass and entry
     std::vector<double> tuned_us[5];
+    std::vector<std::vector<int>> tuned_batches;  // [pass][model space] batch of the two-pass kernels
     // graph
     hipGraphExec_t graph_exec = nullptr;
     double graph_dt = 0;
@@ -954,13 +967,28 @@ class Engine : public EngineBase {
         for (const Pass &p : v) run_transform(kind, p, sel, kk);
     }
 
+    // Cross-stream ordering inside a right-hand side.  Every wait refers to a
+    // record made earlier in the same right-hand side: a record made while a
+    // graph was captured names nodes of that capture's graph, which is
+    // destroyed after instantiation, so a wait on such a record (from a later
+    // capture or from eager launches) would refer to freed capture state.
+    std::vector<hipEvent_t> rhs_records;
+    void record(hipEvent_t ev, hipStream_t s) {
+        HIPCHECK(hipEventRecord(ev, s));
+        rhs_records.push_back(ev);
+    }
+    void wait(hipStream_t s, hipEvent_t ev) {
+        if (std::find(rhs_records.begin(), rhs_records.end(), ev) == rhs_records.end())
+            throw BldError{BLD_EHIP, "internal: stream wait on an event not recorded in this right-hand side"};
+        HIPCHECK(hipStreamWaitEvent(s, ev, 0));
+    }
     void fork(hipEvent_t ev) {
-        HIPCHECK(hipEventRecord(ev, st));
-        HIPCHECK(hipStreamWaitEvent(st2, ev, 0));
+        record(ev, st);
+        wait(st2, ev);
     }
     void join(hipEvent_t ev) {
-        HIPCHECK(hipEventRecord(ev, st2));
-        HIPCHECK(hipStreamWaitEvent(st, ev, 0));
+        record(ev, st2);
+        wait(st, ev);
     }
 
     // compute_rates! of state `sel` (0 = u, 1 = utmp) of every model into du.
@@ -972,7 +1000,9 @@ class Engine : public EngineBase {
     // transforms cover both kinds of space in one launch, so the side stream
     // waits for them.  With per-kernel profiling everything stays on the
     // engine stream.
-    void rhs(int sel) {
+    void rhs(int sel, bool count_step = false) {
+        int *const cnt = count_step ? d_flag.as<int>() + 1 : nullptr;
+        rhs_records.clear();
         const bool two_pass_c2r = !tp[PASS_C2R_X2].empty();
         bool any_2d = false;
         for (auto &g : tp_c2r2d)
@@ -994,10 +1024,10 @@ class Engine : public EngineBase {
         if (side) {
             fork(ev_fork);
             c2r2d(0, st2);
-            launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st);
-            HIPCHECK(hipEventRecord(ev_lin, st));
+            launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, cnt, st);
+            record(ev_lin, st);
         } else {
-            timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, st); });
+            timed(KK_LINEAR, [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, sel, cnt, st); });
             c2r2d(0, st);
         }
         if (!any_c2r) return;
@@ -1009,7 +1039,7 @@ class Engine : public EngineBase {
         if (side && pp[PHYS_ADV].nblocks) {
             if (two_pass_c2r) fork(ev_fork2);
             launch_physics<T>(PHYS_ADV, dsp, pp[PHYS_ADV].lt(), pp[PHYS_ADV].nblocks, st2);
-            HIPCHECK(hipStreamWaitEvent(st2, ev_lin, 0));  // du holds the linear terms
+            wait(st2, ev_lin);  // du holds the linear terms
             for (const Pass &p : tp_r2c_2d)
                 launch_transform<T>(PASS_R2C_2D, dsp, p.lt(), p.nblocks, p.smem, sel, st2);
             adv_r2c_done = true;
@@ -1117,17 +1147,17 @@ class Engine : public EngineBase {
         switch (method) {
             case BLD_EULER:
                 stage(STAGE_EULER, dt);
-                rhs(0);
+                rhs(0, true);
                 break;
             case BLD_SSPRK22:
                 stage(STAGE_RK_1, dt);
                 rhs(1);
                 stage(STAGE_RK2_2, dt);
-                rhs(0);
+                rhs(0, true);
                 break;
             case BLD_AB2:
                 stage(STAGE_AB2, dt);
-                rhs(0);
+                rhs(0, true);
                 break;
             default:
                 stage(STAGE_RK_1, dt);
@@ -1135,7 +1165,7 @@ class Engine : public EngineBase {
                 stage(STAGE_RK3_2, dt);
                 rhs(1);
                 stage(STAGE_RK3_3, dt);
-                rhs(0);
+                rhs(0, true);
         }
         sample_profiles(0.5 * dt);
     }
@@ -1151,7 +1181,7 @@ class Engine : public EngineBase {
         stage(STAGE_AB2_FIRST_1, dt);
         rhs(0);
         stage(STAGE_AB2_FIRST_2, dt);
-        rhs(0);
+        rhs(0, true);
         sample_profiles(0.5 * dt);
         for (auto &I : inst) I->prof_steps += I->prof_mask ? 1 : 0;
         ab2_started = true;
@@ -1167,13 +1197,16 @@ class Engine : public EngineBase {
         ab2_started = false;
     }
 
+    // step count at the first non-finite rate since clear_flag(), -1 if none
     int read_flag() {
-        int flag = 0;
-        HIPCHECK(hipMemcpyAsync(&flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(h_flag, d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        return flag;
+        return h_flag[0] == INT_MAX ? -1 : h_flag[0];
     }
-    void clear_flag() { HIPCHECK(hipMemsetAsync(d_flag.p, 0, sizeof(int), st)); }
+    void clear_flag() {
+        static const int init[2] = {INT_MAX, 0};
+        HIPCHECK(hipMemcpyAsync(d_flag.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+    }
 
     // -- state transfer (selected model) ------------------------------------------
     void set_state(const double *const f[3]) override {
@@ -1229,7 +1262,7 @@ class Engine : public EngineBase {
         check_du();
         const int flag = read_flag();
         download(cur(), cur().d_du, cur().d_du00, true, f);
-        return flag ? BLD_EDIVERGED : BLD_OK;
+        return flag >= 0 ? BLD_EDIVERGED : BLD_OK;
     }
 
     void project() override {
@@ -1244,7 +1277,7 @@ class Engine : public EngineBase {
         rhs(0);
         sample_profiles(0.0);
         check_du();
-        return read_flag() ? BLD_EDIVERGED : BLD_OK;
+        return read_flag() >= 0 ? BLD_EDIVERGED : BLD_OK;
     }
 
     void capture(double dt) {
@@ -1265,23 +1298,58 @@ class Engine : public EngineBase {
         graph_dt = dt;
     }
 
-    int step(double dt, int64_t n, bool graph) override {
+    // Every rate is checked by the kernel that consumes it (the stage kernels;
+    // the trailing rate of the last step by one check kernel), as the
+    // reference checks after each rate! (BoundaryLayerDynamics.jl:155-157).
+    // Steps go in chunks; the flag of each chunk is copied to pinned memory
+    // behind it and read while the next chunk runs, so the loop ends at most
+    // one chunk after the first non-finite rate, and *done is the number of
+    // time advances before it (ODEMethods.jl:217-231: the time is updated
+    // after perform_step! and before the trailing rate!).  The state after a
+    // divergence holds whatever the stage kernels made of the non-finite
+    // rates (the reference keeps the last completed step).
+    int step(double dt, int64_t n, bool graph, int64_t *done) override {
         clear_flag();
+        const int64_t n0 = n;
         n = ab2_start(dt, n);
         if (graph && n > 0) capture(dt);
-        for (int64_t i = 0; i < n; ++i) {
-            if (graph)
-                HIPCHECK(hipGraphLaunch(graph_exec, st));
-            else
-                one_step(dt);
+        const int64_t chunk = 32;
+        int64_t launched = 0, k = 0;
+        int first = -1;
+        auto chunk_flag = [&](int64_t c) {  // flag copied behind chunk c
+            HIPCHECK(hipEventSynchronize(ev_chunk[c % 2]));
+            const int v = h_flag[2 * (c % 2)];
+            return v == INT_MAX ? -1 : v;
+        };
+        for (; launched < n; ++k) {
+            const int64_t m = std::min(chunk, n - launched);
+            for (int64_t i = 0; i < m; ++i) {
+                if (graph)
+                    HIPCHECK(hipGraphLaunch(graph_exec, st));
+                else
+                    one_step(dt);
+            }
+            launched += m;
+            if (launched == n) check_du();
+            HIPCHECK(hipMemcpyAsync(h_flag + 2 * (k % 2), d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipEventRecord(ev_chunk[k % 2], st));
+            if (k > 0 && (first = chunk_flag(k - 1)) >= 0) break;
         }
-        for (auto &I : inst) I->prof_steps += I->prof_mask ? n : 0;
-        check_du();
+        if (first < 0 && k > 0) first = chunk_flag(k - 1);
+        if (n == 0 && n0 > 0) {  // the Adams–Bashforth start step alone
+            check_du();
+            first = read_flag();
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        // the step count includes the Adams–Bashforth start step
+        *done = first >= 0 ? std::min<int64_t>(first, n0) : n0;
+        for (auto &I : inst) I->prof_steps += I->prof_mask ? *done : 0;
         if (profiling) collect_profile();
-        return read_flag() ? BLD_EDIVERGED : BLD_OK;
+        return first >= 0 ? BLD_EDIVERGED : BLD_OK;
     }
 
-    double time_steps(double dt, int64_t n, bool graph) override {
+    double time_steps(double dt, int64_t n, bool graph, bool *diverged) override {
+        clear_flag();
         n = ab2_start(dt, n);
         if (graph) capture(dt);
         hipEvent_t a, b;
@@ -1295,7 +1363,8 @@ class Engine : public EngineBase {
                 one_step(dt);
         }
         HIPCHECK(hipEventRecord(b, st));
-        HIPCHECK(hipEventSynchronize(b));
+        check_du();  // the trailing rate of the last step, after the timed region
+        *diverged = read_flag() >= 0;
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, a, b));
         (void)hipEventDestroy(a);
@@ -1345,7 +1414,7 @@ class Engine : public EngineBase {
             case KK_R2C_X2: add(PASS_R2C_X2, tp[PASS_R2C_X2]); add(PASS_R2C_X2, tp_r2c_x2_sgs); break;
             case KK_LINEAR:
                 n_other = 1;
-                other = [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, 0, st); };
+                other = [&] { launch_linear<T>(d_lin.as<LinDesc<T>>(), lin_pass.lt(), lin_pass.nblocks, 0, nullptr, st); };
                 break;
             case KK_ADV:
             case KK_SGS_VISC:
@@ -1508,8 +1577,22 @@ class Engine : public EngineBase {
             tuned_mode = bm;
             tuned_us_rhs = bt;
         }
-        // chunk length of the stage kernel: stage 1 only writes utmp, so it can be
-        // replayed on the live state
+        tuned_batches = best;
+        // chunk length of the stage kernel: stage 1 only writes utmp (and the
+        // float64 (0,0) shadow of utmp), so it can be replayed on the live state
+        // with utmp saved before and restored after (the Adams–Bashforth
+        // method keeps f(u) of the previous step there)
+        std::vector<DevBuf> saved;
+        auto save = [&](DevBuf &src) {
+            saved.emplace_back();
+            saved.back().alloc(src.bytes);
+            if (src.bytes) HIPCHECK(hipMemcpyAsync(saved.back().p, src.p, src.bytes, hipMemcpyDeviceToDevice, st));
+        };
+        for (auto &I : inst)
+            for (int f = 0; f < 3; ++f) {
+                save(I->d_ut[f]);
+                if (I->mixed) save(I->d_ut00[f]);
+            }
         int n3max = 0;
         for (auto &I : inst) n3max = std::max(n3max, I->n3);
         double best_us = 1e30;
@@ -1518,11 +1601,26 @@ class Engine : public EngineBase {
             if (n3max > 64 * cl) continue;
             stage_CL_tuned = cl;
             build_point_plans();
-            const double us = time_us([&] { stage(1, 1e-6); }, reps);
+            const double us = time_us([&] { stage(STAGE_RK_1, 1e-6); }, reps);
             if (us < best_us) {
                 best_us = us;
                 best_cl = cl;
             }
+        }
+        {
+            size_t i = 0;
+            for (auto &I : inst)
+                for (int f = 0; f < 3; ++f) {
+                    DevBuf &a = I->d_ut[f];
+                    if (a.bytes) HIPCHECK(hipMemcpyAsync(a.p, saved[i].p, a.bytes, hipMemcpyDeviceToDevice, st));
+                    ++i;
+                    if (I->mixed) {
+                        DevBuf &b = I->d_ut00[f];
+                        if (b.bytes) HIPCHECK(hipMemcpyAsync(b.p, saved[i].p, b.bytes, hipMemcpyDeviceToDevice, st));
+                        ++i;
+                    }
+                }
+            HIPCHECK(hipStreamSynchronize(st));
         }
         stage_CL_tuned = best_cl;
         tuned_us[4].assign(1, best_us);
@@ -1533,6 +1631,22 @@ class Engine : public EngineBase {
         }
         rhs(0);  // restore du = f(u)
         HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // Forces the transform path (the autotuner's choice, bit 0: whole-plane
+    // c2r, bit 1: whole-plane r2c, bit 2: half-plane c2r blocks where a whole
+    // plane needs 1024-thread blocks); spaces a path does not fit keep the
+    // two-pass transforms.  Results do not depend on the path.
+    void set_transform_path(int path) override {
+        if (path < 0 || path > 7) throw BldError{BLD_EARG, "unknown transform path"};
+        drop_graph();
+        HIPCHECK(hipStreamSynchronize(st));
+        fused_c2r = (path & 1) != 0;
+        fused_r2c = (path & 2) != 0;
+        half_c2r = (path & 4) != 0;
+        tuned_mode = path;
+        build_transform_passes(tuned_batches.empty() ? nullptr : &tuned_batches);
+        HIPCHECK(hipDeviceSynchronize());
     }
 
     void tuning(int pass, int entry, int *batch, double *us) override {
@@ -1915,7 +2029,9 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
             const int l = l0 + j;
             const cplx<A> xm = j > 0 ? xs[j - 1] : below;
             const cplx<A> xp = j + 1 < CL ? xs[j + 1] : above;
-            const double blo = l > 0 ? double(gload(beta + size_t(max(l - 1, 0)) * nk + mm)) : 0.0;
+            // levels past nz (the last chunk's padding) load at a clamped level
+            // and are discarded like every other value outside the column
+            const double blo = l > 0 ? double(gload(beta + size_t(min(max(l - 1, 0), max(nz - 2, 0))) * nk + mm)) : 0.0;
             const double bup = l + 1 < nz ? double(gload(beta + size_t(min(l, max(nz - 2, 0))) * nk + mm)) : 0.0;
             const double rx = double(r[j].x) - (d[j] * double(xs[j].x) + blo * double(xm.x) + (l + 1 < nz ? bup * double(xp.x) : 0.0));
             const double ry = double(r[j].y) - (d[j] * double(xs[j].y) + blo * double(xm.y) + (l + 1 < nz ? bup * double(xp.y) : 0.0));
@@ -2072,6 +2188,13 @@ int bld_autotune(bld_engine *e, int reps) {
     })
 }
 
+int bld_set_transform_path(bld_engine *e, int path) {
+    ENGINE_CALL({
+        e->impl->set_transform_path(path);
+        return BLD_OK;
+    })
+}
+
 int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us) {
     ENGINE_CALL({
         if (!batch || !us) throw BldError{BLD_EARG, "null output"};
@@ -2133,8 +2256,9 @@ static void advance_time(double *t, double dt, int64_t n, double t_end) {
 int bld_step(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end) {
     ENGINE_CALL({
         if (nsteps < 0) throw BldError{BLD_EARG, "negative step count"};
-        int r = e->impl->step(dt, nsteps, false);
-        advance_time(t, dt, nsteps, t_end);
+        int64_t done = 0;
+        int r = e->impl->step(dt, nsteps, false, &done);
+        advance_time(t, dt, done, t_end);
         if (r == BLD_EDIVERGED) e->err = "The simulation has diverged.";
         return r;
     })
@@ -2143,8 +2267,9 @@ int bld_step(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end) 
 int bld_step_graph(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end) {
     ENGINE_CALL({
         if (nsteps < 0) throw BldError{BLD_EARG, "negative step count"};
-        int r = e->impl->step(dt, nsteps, true);
-        advance_time(t, dt, nsteps, t_end);
+        int64_t done = 0;
+        int r = e->impl->step(dt, nsteps, true, &done);
+        advance_time(t, dt, done, t_end);
         if (r == BLD_EDIVERGED) e->err = "The simulation has diverged.";
         return r;
     })
@@ -2169,7 +2294,12 @@ int bld_get_field_physical(bld_engine *e, int field, double *values, int centere
 int bld_time_steps(bld_engine *e, double dt, int64_t nsteps, int use_graph, double *ms) {
     ENGINE_CALL({
         if (!ms || nsteps < 0) throw BldError{BLD_EARG, "invalid arguments"};
-        *ms = e->impl->time_steps(dt, nsteps, use_graph != 0);
+        bool diverged = false;
+        *ms = e->impl->time_steps(dt, nsteps, use_graph != 0, &diverged);
+        if (diverged) {
+            e->err = "The simulation has diverged.";
+            return BLD_EDIVERGED;
+        }
         return BLD_OK;
     })
 }

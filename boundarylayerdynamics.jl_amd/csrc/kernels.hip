@@ -17,6 +17,10 @@
 // gradient (ODEMethods.jl:147-166, pressure.jl:334-426).
 #include "kernels.hpp"
 
+#ifndef BLD_KERNEL_PART
+#define BLD_KERNEL_PART 0
+#endif
+
 #include "fft.hpp"
 #include "terms.hpp"
 #include "transforms_2d.hpp"
@@ -590,7 +594,11 @@ __device__ __forceinline__ cplx<A> ghost(int kind, double value, double differen
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ descs, LaunchTable lt, int sel) {
+__global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ descs, LaunchTable lt, int sel,
+                                                int *step_count) {
+    // the trailing rate evaluation of a step counts the step (read by the
+    // divergence records of the kernels ordered after this one)
+    if (step_count && blockIdx.x == 0 && threadIdx.x == 0) step_count[0] += 1;
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
     const LinDesc<T> &L = descs[E.space];
     const int nk = L.nk, n3 = L.n3;
@@ -660,6 +668,13 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
     if (m == 0) res.x += src;
     gstore(du + i, res);
 }
+
+// Divergence record (the isfinite check of BoundaryLayerDynamics.jl:155-157):
+// flag[1] counts the steps whose trailing rate evaluation has started (bumped
+// by the linear-term kernel of that evaluation), so a non-finite rate found by
+// the kernel that consumes it is recorded as the number of time advances made
+// before the reference would raise; flag[0] keeps the smallest such count.
+__device__ __forceinline__ void record_diverged(int *flag) { atomicMin(flag, flag[1]); }
 
 // ---------------------------------------------------------------------------
 // SSPRK33 stage combination + pressure projection, one thread per (k1,k2)
@@ -742,7 +757,7 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
                 if (l < n3 - 1) store(2, l, comb(2, l));
             }
         }
-        if (bad) *D.flag = 1;
+        if (bad) record_diverged(D.flag);
         return;
     }
     const int k1 = m % D.K1, i2 = m / D.K1;
@@ -807,7 +822,7 @@ __global__ void __launch_bounds__(64) k_stage_project(ProjDesc<T> D, int stage, 
         }
         pnext = p;
     }
-    if (bad) *D.flag = 1;
+    if (bad) record_diverged(D.flag);
 }
 
 
@@ -887,7 +902,7 @@ __device__ __forceinline__ void stage_chunked_body(const ProjDesc<T> &D, int blk
                 else c3[j] = c;
             }
         }
-        if (bad) *D.flag = 1;
+        if (bad) record_diverged(D.flag);
     }
     if (!D.has_pressure) {
         if (stage > 0 && active) {
@@ -1100,7 +1115,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_check_finite(const cplx<T> *a, size_t n, int *flag) {
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
         const cplx<T> v = a[i];
-        if (!(isfinite(v.x) && isfinite(v.y))) *flag = 1;
+        if (!(isfinite(v.x) && isfinite(v.y))) record_diverged(flag);
     }
 }
 
@@ -1113,36 +1128,70 @@ __global__ void __launch_bounds__(256) k_fill(cplx<T> *a, size_t n) {
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// One transform kernel kind (PassKind) over the blocks of a launch table.
+template <typename T, int KIND>
+void launch_xf(const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel, hipStream_t st) {
+    if constexpr (KIND == PASS_C2R_X2) k_c2r_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt, sel);
+    else if constexpr (KIND == PASS_C2R_X1) k_c2r_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt);
+    else if constexpr (KIND == PASS_R2C_X1) k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt);
+    else if constexpr (KIND == PASS_R2C_X2) k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt);
+    else if constexpr (KIND == PASS_C2R_2D) k_c2r_2d<T, 1024><<<nblocks, 1024, smem, st>>>(spaces_dev, lt, sel);
+    else if constexpr (KIND == PASS_C2R_2D_HALF) k_c2r_2d<T, 512><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel);
+    else if constexpr (KIND == PASS_C2R_2H) k_c2r_2h<T><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel);
+    else k_r2c_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt);
+}
+
+// the fused transforms keep a whole plane in LDS: allow up to 160 KiB per block
+template <typename T, int KIND>
+hipError_t xf_allow_large_lds() {
+    const void *f = nullptr;
+    if constexpr (KIND == PASS_C2R_2D) f = reinterpret_cast<const void *>(&k_c2r_2d<T, 1024>);
+    else if constexpr (KIND == PASS_C2R_2D_HALF) f = reinterpret_cast<const void *>(&k_c2r_2d<T, 512>);
+    else if constexpr (KIND == PASS_C2R_2H) f = reinterpret_cast<const void *>(&k_c2r_2h<T>);
+    else if constexpr (KIND == PASS_R2C_2D) f = reinterpret_cast<const void *>(&k_r2c_2d<T>);
+    if (!f) return hipSuccess;
+    return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+// Each transform kind is compiled in a translation unit of its own (the
+// BLD_KERNEL_PART builds of this file), so the unrolled FFT bodies of all
+// lengths build in parallel.
+#define BLD_XF_DECL(T, K)                                                                                      \
+    extern template void launch_xf<T, K>(const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
+    extern template hipError_t xf_allow_large_lds<T, K>();
+#define BLD_XF_INST(T, K)                                                                               \
+    template void launch_xf<T, K>(const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
+    template hipError_t xf_allow_large_lds<T, K>();
+
+#if BLD_KERNEL_PART == 0
+#define BLD_XF_ALL(M, T) \
+    M(T, PASS_C2R_X2) M(T, PASS_C2R_X1) M(T, PASS_R2C_X1) M(T, PASS_R2C_X2) M(T, PASS_C2R_2D) M(T, PASS_C2R_2D_HALF) \
+    M(T, PASS_C2R_2H) M(T, PASS_R2C_2D)
+BLD_XF_ALL(BLD_XF_DECL, float)
+BLD_XF_ALL(BLD_XF_DECL, double)
+
 template <typename T>
 void launch_transform(int kind, const Space<T> *spaces_dev, const LaunchTable &lt, int nblocks, size_t smem, int sel,
                       hipStream_t st) {
     if (nblocks <= 0) return;
     switch (kind) {
-        case PASS_C2R_X2: k_c2r_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt, sel); break;
-        case PASS_C2R_X1: k_c2r_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
-        case PASS_R2C_X1: k_r2c_x1<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
-        case PASS_R2C_X2: k_r2c_x2<T><<<nblocks, 256, smem, st>>>(spaces_dev, lt); break;
-        case PASS_C2R_2D: k_c2r_2d<T, 1024><<<nblocks, 1024, smem, st>>>(spaces_dev, lt, sel); break;
-        case PASS_C2R_2D_HALF: k_c2r_2d<T, 512><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel); break;
-        case PASS_C2R_2H: k_c2r_2h<T><<<nblocks, 512, smem, st>>>(spaces_dev, lt, sel); break;
-        case PASS_R2C_2D: k_r2c_2d<T><<<nblocks, kC2r2dThreads, smem, st>>>(spaces_dev, lt); break;
+        case PASS_C2R_X2: launch_xf<T, PASS_C2R_X2>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_C2R_X1: launch_xf<T, PASS_C2R_X1>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_R2C_X1: launch_xf<T, PASS_R2C_X1>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_R2C_X2: launch_xf<T, PASS_R2C_X2>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_C2R_2D: launch_xf<T, PASS_C2R_2D>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_C2R_2D_HALF: launch_xf<T, PASS_C2R_2D_HALF>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_C2R_2H: launch_xf<T, PASS_C2R_2H>(spaces_dev, lt, nblocks, smem, sel, st); break;
+        case PASS_R2C_2D: launch_xf<T, PASS_R2C_2D>(spaces_dev, lt, nblocks, smem, sel, st); break;
     }
 }
 
-// the fused transforms keep a whole plane in LDS: allow up to 160 KiB per block
 template <typename T>
 hipError_t allow_large_lds() {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T, 1024>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2d<T, 512>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_c2r_2h<T>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_r2c_2d<T>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    for (hipError_t e : {xf_allow_large_lds<T, PASS_C2R_2D>(), xf_allow_large_lds<T, PASS_C2R_2D_HALF>(),
+                         xf_allow_large_lds<T, PASS_C2R_2H>(), xf_allow_large_lds<T, PASS_R2C_2D>()})
+        if (e != hipSuccess) return e;
+    return hipSuccess;
 }
 
 template <typename T>
@@ -1161,8 +1210,9 @@ void launch_constant_mean(const CMeanDesc<T> &D, int stage, hipStream_t st) {
 }
 
 template <typename T>
-void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st) {
-    if (nblocks > 0) k_linear<T><<<nblocks, 256, 0, st>>>(descs_dev, lt, sel);
+void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, int *step_count,
+                   hipStream_t st) {
+    if (nblocks > 0) k_linear<T><<<nblocks, 256, 0, st>>>(descs_dev, lt, sel, step_count);
 }
 
 // One launch of the chunked stage kernel over all descriptors of the table;
@@ -1204,7 +1254,7 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
 #define BLD_INSTANTIATE(T)                                                                                         \
     template void launch_transform<T>(int, const Space<T> *, const LaunchTable &, int, size_t, int, hipStream_t); \
     template void launch_physics<T>(int, const Space<T> *, const LaunchTable &, int, hipStream_t);                \
-    template void launch_linear<T>(const LinDesc<T> *, const LaunchTable &, int, int, hipStream_t);               \
+    template void launch_linear<T>(const LinDesc<T> *, const LaunchTable &, int, int, int *, hipStream_t);               \
     template void launch_constant_mean<T>(const CMeanDesc<T> &, int, hipStream_t);                                \
     template void launch_stage_chunked<T>(int, const ProjDesc<T> *, const LaunchTable &, int, int, double, hipStream_t, \
                                           const ProjDesc<double> *, int);                                        \
@@ -1215,5 +1265,23 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
 
 BLD_INSTANTIATE(float)
 BLD_INSTANTIATE(double)
+
+#elif BLD_KERNEL_PART == 1
+BLD_XF_INST(float, PASS_C2R_X2) BLD_XF_INST(float, PASS_C2R_X1)
+#elif BLD_KERNEL_PART == 2
+BLD_XF_INST(float, PASS_R2C_X1) BLD_XF_INST(float, PASS_R2C_X2)
+#elif BLD_KERNEL_PART == 3
+BLD_XF_INST(float, PASS_C2R_2D) BLD_XF_INST(float, PASS_C2R_2D_HALF)
+#elif BLD_KERNEL_PART == 4
+BLD_XF_INST(float, PASS_C2R_2H) BLD_XF_INST(float, PASS_R2C_2D)
+#elif BLD_KERNEL_PART == 5
+BLD_XF_INST(double, PASS_C2R_X2) BLD_XF_INST(double, PASS_C2R_X1)
+#elif BLD_KERNEL_PART == 6
+BLD_XF_INST(double, PASS_R2C_X1) BLD_XF_INST(double, PASS_R2C_X2)
+#elif BLD_KERNEL_PART == 7
+BLD_XF_INST(double, PASS_C2R_2D) BLD_XF_INST(double, PASS_C2R_2D_HALF)
+#elif BLD_KERNEL_PART == 8
+BLD_XF_INST(double, PASS_C2R_2H) BLD_XF_INST(double, PASS_R2C_2D)
+#endif
 
 }  // namespace bld

@@ -47,7 +47,7 @@ struct ProjDesc {
     cplx<double> *pbuf;                 // [n3][nk] scratch
     cplx<T> *u[3], *ut[3], *du[3];
     cplx<double> *u00[3], *ut00[3], *du00[3];
-    int *flag;
+    int *flag;                          // [0] first diverged step count, [1] step count (record_diverged)
     // float64 (0,0)-mode pass of the mixed form: the float32 nonlinear rate of
     // the mode is added to du, and results are mirrored into the float32 state
     int MB, skip0;                      // chunked kernel: modes per block; skip the (0,0) column
@@ -121,7 +121,8 @@ void launch_closure_sample(const Space<T> *space_dev, int N1, int N2, int n3, in
 template <typename T>
 void launch_constant_mean(const CMeanDesc<T> &D, int stage, hipStream_t st);
 template <typename T>
-void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, hipStream_t st);
+void launch_linear(const LinDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int sel, int *step_count,
+                   hipStream_t st);
 template <typename T>
 void launch_stage_chunked(int CL, const ProjDesc<T> *descs_dev, const LaunchTable &lt, int nblocks, int stage,
                           double dt, hipStream_t st, const ProjDesc<double> *descs00_dev = nullptr, int n00 = 0);

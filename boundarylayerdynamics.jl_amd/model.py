@@ -661,9 +661,16 @@ class Model:
         self._check(self._lib.bld_set_method(self._h, code))
 
     def step(self, dt, nsteps, t=0.0, t_end=0.0, graph=True):
+        """nsteps × (perform_step! + rate!); returns the new time.  A
+        non-finite rate raises "The simulation has diverged." with the time the
+        reference reaches before it raises as the error's ``t``."""
         tt = ctypes.c_double(t)
         fn = self._lib.bld_step_graph if graph else self._lib.bld_step
-        self._check(fn(self._h, dt, nsteps, ctypes.byref(tt), t_end))
+        rc = fn(self._h, dt, nsteps, ctypes.byref(tt), t_end)
+        if rc != 0:
+            err = BldError(rc, self._lib.bld_last_error(self._h).decode())
+            err.t = tt.value
+            raise err
         return tt.value
 
     def time_steps(self, dt, nsteps, graph=True):
@@ -674,6 +681,15 @@ class Model:
     def autotune(self, reps=5):
         """FFT batch autotuner (bld_autotune); recomputes the rates of the state."""
         self._check(self._lib.bld_autotune(self._h, reps))
+
+    TRANSFORM_PATHS = {"two-pass": 0, "whole-plane": 3, "half-plane": 7, "whole-plane-c2r": 1, "whole-plane-r2c": 2}
+
+    def set_transform_path(self, path):
+        """Force the transform path of every model space (bld_set_transform_path):
+        "whole-plane" (c2r and r2c, 1024-thread blocks for the largest planes),
+        "half-plane" (whole-plane, with half-plane c2r blocks where a plane needs
+        1024 threads), "two-pass", or only one direction whole-plane."""
+        self._check(self._lib.bld_set_transform_path(self._h, self.TRANSFORM_PATHS[path]))
 
     def tuning(self):
         """{pass: [(batch, µs) per model space]} chosen by the autotuner."""

@@ -129,6 +129,13 @@ int bld_autotune(bld_engine *e, int reps);
  * that fit in LDS; bit 2: c2r of the largest planes by half planes) and the
  * time of one right-hand side (entry ignored). */
 int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us);
+/* Transform path forced for testing and measurement (no reference
+ * counterpart): the pass-5 bits of bld_tuning (bit 0 whole-plane c2r, bit 1
+ * whole-plane r2c, bit 2 half-plane c2r blocks for planes that need
+ * 1024-thread whole-plane blocks).  Model spaces a path does not fit keep the
+ * two-pass transforms; the autotuned batch sizes are kept.  Results do not
+ * depend on the path. */
+int bld_set_transform_path(bld_engine *e, int path);
 
 /* Learned SGS closure (no reference counterpart; it replaces the stresses of
  * the selected model's StaticSmagorinskyModel, static_smagorinsky.jl:129-249,
@@ -195,8 +202,13 @@ int bld_init_problem(bld_engine *e, double t1);
 
 /* nsteps × (perform_step!(SSPRK33) + rate!) (src/ODEMethods.jl:147-166,
  * 226-231).  *t is advanced with the reference's snapping rule when t_end is
- * given (t_end <= *t disables snapping).  Returns BLD_EDIVERGED if any rate
- * became non-finite. */
+ * given (t_end <= *t disables snapping).  Every rate is checked for non-finite
+ * values (src/BoundaryLayerDynamics.jl:155-157); at the first one the call
+ * returns BLD_EDIVERGED with *t advanced only by the time updates the
+ * reference makes before it raises (the steps completed before the step whose
+ * inner rate diverged, or including the step whose trailing rate diverged).
+ * The state is then unspecified (the reference keeps the last completed
+ * step). */
 int bld_step(bld_engine *e, double dt, int64_t nsteps, double *t, double t_end);
 
 /* Same as bld_step but records the whole step sequence into a HIP graph the
@@ -212,7 +224,9 @@ int bld_set_field_physical(bld_engine *e, int field, const double *values, int c
 int bld_get_field_physical(bld_engine *e, int field, double *values, int centered);
 
 /* Engine timing hook: runs nsteps steps with inputs resident, returns the
- * elapsed milliseconds measured with HIP events on the engine stream. */
+ * elapsed milliseconds measured with HIP events on the engine stream.  The
+ * rates are checked inside the timed steps as in bld_step; the call returns
+ * BLD_EDIVERGED (with *ms set) if one was non-finite. */
 int bld_time_steps(bld_engine *e, double dt, int64_t nsteps, int use_graph, double *ms);
 
 /* Per-kernel timing: average duration (ms) of each kernel class over the last

@@ -60,8 +60,10 @@ CASES = {
                        upper="FreeSlipBoundary",
                        procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
                                                              sgs_model=M.StaticSmagorinskyModel(Cs=0.1))),
-    # larger square LES grids: advection 96x96 (fp64: half-plane c2r blocks) and
-    # 144x144 (fp32: half-plane c2r blocks)
+    # larger square LES grids: advection 96x96 (fp64: half-plane c2r blocks)
+    # and 142x142 (the quadratic rule at k = 47 gives 3k+1 = 142 = 2·71, which
+    # has no register FFT: two-pass LDS transforms); the headline's 192² and
+    # 128² planes are covered in test_gpu_headline.py
     "les_64": dict(res=(64, 64, 6), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4),
                    upper="FreeSlipBoundary",
                    procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
@@ -223,6 +225,60 @@ def test_divergence_detected():
     gpu.set_state(**st)
     with pytest.raises(B._lib.BldError, match="diverged"):
         gpu.rates()
+
+
+def oracle_divergence_time(ref, dt, nmax):
+    """Time of the oracle's problem when its rate! raises "diverged"
+    (ODEMethods.jl:217-231, BoundaryLayerDynamics.jl:155-157), or None."""
+    import oracle.model as OM
+    u0 = [ref.state[f] for f in ref.fields]
+    prob = O.ODEProblem(OM.rate_fn(ref), u0, (0.0, nmax * dt), projection=OM.projection_fn(ref), checkpoint=True)
+    try:
+        with np.errstate(all="ignore"):
+            O.solve(prob, O.SSPRK33(), dt)
+    except FloatingPointError:
+        return prob.t
+    return None
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_divergence_reported_at_the_reference_step(graph):
+    """A state far beyond the advective stability limit overflows within a
+    few steps: the step call stops there with "diverged" and the time the
+    reference's problem has reached when it raises (every rate is checked by
+    the kernel that consumes it, inside the captured graph too)."""
+    ref, gpu = pair("channel", seed=11, amp=3.0)
+    dt, n = 0.05, 100
+    t_ref = oracle_divergence_time(ref, dt, n)
+    assert t_ref is not None and t_ref < 10 * dt
+    gpu.init_problem()
+    with pytest.raises(B._lib.BldError, match="The simulation has diverged.") as ei:
+        gpu.step(dt, n, t=0.0, t_end=n * dt, graph=graph)
+    assert abs(ei.value.t - t_ref) < 1e-9 * dt, (ei.value.t, t_ref)
+
+
+def test_divergence_after_several_step_chunks():
+    """A slower blow-up (well over 32 steps, the length of the step chunks
+    after which the flag is read): graph replays and single launches stop at
+    the same step, near the oracle's (the trajectories separate by rounding
+    before the overflow), and the timing hook reports the divergence too."""
+    dt, n = 0.05, 400
+    ts = []
+    for graph in (False, True):
+        ref, gpu = pair("channel", seed=11, amp=1.0)
+        gpu.init_problem()
+        with pytest.raises(B._lib.BldError, match="diverged") as ei:
+            gpu.step(dt, n, t=0.0, t_end=n * dt, graph=graph)
+        ts.append(ei.value.t)
+    assert ts[0] == ts[1], ts
+    assert 40 * dt < ts[0] < n * dt, ts
+    ref, _ = pair("channel", seed=11, amp=1.0)
+    t_ref = oracle_divergence_time(ref, dt, n)
+    assert t_ref is not None and abs(ts[0] - t_ref) <= 0.2 * t_ref, (ts, t_ref)
+    _, gpu = pair("channel", seed=11, amp=1.0)
+    gpu.init_problem()
+    with pytest.raises(B._lib.BldError, match="diverged"):
+        gpu.time_steps(dt, n, graph=True)
 
 
 def test_unsupported_configs():
@@ -455,6 +511,24 @@ def test_other_methods_fp32(method):
     got = gpu.state
     for f in ref.fields:
         assert relerr(got[f], ref.state[f]) < 1e-4, (method, f, relerr(got[f], ref.state[f]))
+
+
+@pytest.mark.parametrize("name", ["channel", "les_square"])
+def test_ab2_history_survives_autotune(name):
+    """The autotuner replays the first stage kernel, which writes utmp; the
+    Adams–Bashforth method keeps f(u) of the previous step there, so autotune
+    between AB2 steps must not change the next steps."""
+    ref, gpu = pair(name, seed=13)
+    dt = 1e-3 if name == "channel" else 5e-4
+    O.evolve(ref, 6 * dt, dt, method=O.AB2())
+    gpu.set_method(B.AB2())
+    gpu.init_problem()
+    t = gpu.step(dt, 3, t=0.0, t_end=6 * dt)
+    gpu.autotune(reps=2)
+    gpu.step(dt, 3, t=t, t_end=6 * dt)
+    got = gpu.state
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < RTOL64, (name, f, relerr(got[f], ref.state[f]))
 
 
 def test_ab2_requires_constant_dt():
