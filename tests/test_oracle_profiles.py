@@ -13,7 +13,8 @@ from oracle.model import set_field
 def test_linear_shear_profiles():
     c, z0, kappa = 1.7, 1e-3, 0.4
     dom = O.Domain((2 * math.pi, 2 * math.pi, 1.0), O.RoughWall(z0, kappa), O.FreeSlipBoundary())
-    m = O.Model((8, 8, 6), dom, O.incompressible_flow(1e-3, sgs_model=O.StaticSmagorinskyModel(Cs=0.1)))
+    m = O.Model((8, 8, 6), dom, O.incompressible_flow(1e-3, sgs_model=O.StaticSmagorinskyModel(Cs=0.1,
+                                                                                              wall_damping=False)))
     n1, n2 = O.pdsize(m.grid, None)
     set_field(m.state["vel1"], lambda x, y, z: c * z + 0 * x, dom, m.grid, "C", n1, n2)
     s = P.samples(m, m.state)
@@ -27,7 +28,8 @@ def test_linear_shear_profiles():
     assert s["adv3"][0] == 0 and s["adv3"][-1] == 0
     assert np.allclose(s["adv1"], 0, atol=1e-12) and np.allclose(s["adv2"], 0, atol=1e-12)
     # Smagorinsky at the interior I-nodes: S13 = c/2, |S| = c, τ13 = −2 l² |S| S13 = −l² c²
-    li = [pd for k, pd in m.disc if k == "sgs"][0]["li"]
+    # with the undamped length l = Cs·∛(Δ1Δ2Δ3) = 0.1·∛(π/4·π/4·1/6)
+    li = 0.1 * (math.pi ** 2 / 96) ** (1 / 3)
     assert np.allclose(s["sgs13"][1:-1], -li ** 2 * c * c, rtol=1e-12)
     # wall entry: rough-wall stress −κ²·u·|u|/ln²(z1/z0) with the sign of the lower wall
     z1 = zc[0]
