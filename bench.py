@@ -80,7 +80,7 @@ def time_form(B, precision, res, dt, steps, warmup, profile_steps, autotune=Fals
 def roofline(B, m, kt, precision, res, adv_n, sgs_n):
     sizes = dict(K1=m.k1max + 1, K2=2 * m.k2max + 1, n3=m.n3,
                  spaces=[("adv", *adv_n), ("sgs", *sgs_n)])
-    ab = B.workloads.algorithmic_bytes(sizes, None, precision)
+    ab = B.workloads.algorithmic_bytes(sizes, None, precision, m.tuning()["transform_path"][0])
     # ab[k]: algorithmic bytes of kernel class k per right-hand side (all spaces);
     # a class may take several dispatches per right-hand side (one per space or
     # LDS class), so bytes and durations are averaged per dispatch
@@ -109,6 +109,18 @@ def roofline(B, m, kt, precision, res, adv_n, sgs_n):
     return dict(kernel=dom, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(gbs / HBM_PEAK_GBS, 4), traffic=None, avg_us=round(avg_us, 2),
                 avg_us_event_pairs=round(d["avg_us"], 2), bytes_per_launch=int(bpl)), per
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical core count."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return f"{line.split(':', 1)[1].strip()} ({os.cpu_count()} logical cores visible)"
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(res, dt, budget_s):
@@ -145,9 +157,166 @@ def cpu_baseline(res, dt, budget_s):
         el = time.perf_counter() - t0
         if el > budget_s or nsteps >= 50:
             break
-    return dict(value=round(nsteps / el, 4), unit="steps/s", cores=cores, kind="port",
+    return dict(value=round(nsteps / el, 4), unit="steps/s", cores=cores, cpu_model=cpu_model(), kind="port",
                 sample=f"{nsteps} SSPRK33 steps (+1 warm-up) of the same {n[0]}x{n[1]}x{n[2]} LES in float64, "
                        f"oracle (NumPy + scipy.fft with {cores} workers), {el:.1f} s")
+
+
+def make_dns(B, precision, res=(64, 64, 64)):
+    W = B.workloads
+    n, dom, procs = W.dns_channel(B, res)
+    m = B.Model(n, dom, procs, precision=precision)
+    n1, n2 = m.default_size()
+    u, v, w = W.dns_initial(m.coordinates("vel1", 3), m.coordinates("vel3", 3), n1, n2)
+    for f, vals in zip(("vel1", "vel2", "vel3"), (u, v, w)):
+        m.set_physical(f, vals)
+    m.init_problem(0.0)
+    return m
+
+
+def c1_bench(B, steps=200, warmup=10, profile_steps=5, cpu_seconds=8.0):
+    """The 64³ closed-channel DNS in plain float64 (ν = 1/180, forcing 1,
+    workflow parabola + 5 % seeded noise, SSPRK33 dt = 1e-3, 200 steps): steps/s,
+    grid-point updates/s, the roofline of its dominant kernel class, and the
+    oracle on a bounded sample of the same workload."""
+    res, dt = (64, 64, 64), 1e-3
+    m = make_dns(B, "fp64", res)
+    m.autotune(reps=3)
+    m.step(dt, warmup)
+    ms = m.time_steps(dt, steps)
+    time.sleep(PROFILE_GAP_S)
+    m.set_profiling(True)
+    m.step(dt, profile_steps, graph=False)
+    kt = m.kernel_times()
+    m.set_profiling(False)
+    adv_n = B.pdsize((m.k1max, m.k2max), "quadratic")
+    sizes = dict(K1=m.k1max + 1, K2=2 * m.k2max + 1, n3=m.n3, spaces=[("adv", *adv_n)])
+    ab = B.workloads.algorithmic_bytes(sizes, None, "fp64", m.tuning()["transform_path"][0])
+    n_rhs = kt["linear"][1]
+    per = {}
+    for k, (kms, n) in kt.items():
+        if n == 0 or k not in ab:
+            continue
+        calls = n_rhs if k != "stage_project" else n
+        per[k] = dict(avg_us=1e3 * kms / n, total_ms=kms, bytes_per_launch=ab[k] * calls / n)
+    dom = max(per, key=lambda k: per[k]["total_ms"])
+    avg_us, bpl = per[dom]["avg_us"], per[dom]["bytes_per_launch"]
+    try:
+        us, nd = m.time_kernel(dom, reps=20)
+        if us > 0 and nd > 0:
+            avg_us, bpl = us, ab[dom] / nd
+    except B._lib.BldError:
+        pass
+    gbs = bpl / (avg_us * 1e-6) / 1e9
+    finite = all(np.all(np.isfinite(a)) for a in m.state.values())
+    sps = 1e3 * steps / ms
+    cpu = None
+    if cpu_seconds > 0:
+        sys.path.insert(0, REPO)
+        import oracle as O
+        import oracle.model as OM
+        from oracle.ode import perform_step
+        W = B.workloads
+        n, dom_o, procs = W.dns_channel(O, res)
+        r = O.Model(n, dom_o, procs)
+        n1, n2 = O.pdsize(r.grid)
+        u, v, w = W.dns_initial(r.coordinates("vel1", 3), r.coordinates("vel3", 3), n1, n2)
+        for f, vals in zip(r.fields, (u, v, w)):
+            r.state[f][...] = 0
+            O.add_field(r.state[f], vals)
+        OM.FFT_WORKERS = os.cpu_count() or 1
+        prob = O.ODEProblem(OM.rate_fn(r), [r.state[f] for f in r.fields], (0.0, 1e9),
+                            projection=OM.projection_fn(r))
+        buf = {"utmp": [np.zeros_like(x) for x in prob.u]}
+        perform_step(prob, dt, O.SSPRK33(), buf)
+        prob.rate(prob.du, prob.u, dt)
+        k, t0 = 0, time.perf_counter()
+        while True:
+            perform_step(prob, dt, O.SSPRK33(), buf)
+            prob.rate(prob.du, prob.u, dt)
+            k += 1
+            el = time.perf_counter() - t0
+            if el > cpu_seconds or k >= 200:
+                break
+        cpu = dict(value=round(k / el, 3), unit="steps/s", cores=os.cpu_count(), kind="port",
+                   sample=f"{k} SSPRK33 steps (+1 warm-up) of the 64x64x64 DNS in float64, oracle, {el:.1f} s")
+    return dict(workload="closed-channel DNS 64x64x64 fp64, SSPRK33 dt=1e-3", steps=steps,
+                steps_per_s=round(sps, 2), ms_per_step=round(ms / steps, 4),
+                grid_point_updates_per_s=round(sps * res[0] * res[1] * res[2], 1), finite=finite,
+                transform_path=m.tuning()["transform_path"],
+                roofline=dict(kernel=dom, bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(gbs / HBM_PEAK_GBS, 4), avg_us=round(avg_us, 2), bytes_per_launch=int(bpl)),
+                step_bytes_fraction=round(17 * 16 * sizes["K1"] * sizes["K2"] * (3 * m.n3 - 1) * sps / 8e12, 4),
+                kernels={k: round(v["avg_us"], 2) for k, v in per.items()}, cpu_baseline=cpu)
+
+
+def _arr_of(disc, i, n):
+    """The i-th float64 array the configuration of `disc` points to (0: α at
+    C-nodes [n3], 1: α at I-nodes incl. the walls [n3+1])."""
+    a = disc._keep[i]
+    assert a.shape == (n,)
+    return a
+
+
+def pressure_of(B, m, state, rates, dt):
+    """The pressure that the first stage of the next SSPRK33 step solves for:
+    the projection of u + dt·f(u) (pressure.jl:300-426), from the state and
+    rates of `m` read back in float64, solved with the engine's batched LDLt
+    in float64.  The scaled pressure of the reference (αc⁻¹-scaled system)."""
+    d = m.disc
+    n3, K1, K2 = m.n3, m.k1max + 1, 2 * m.k2max + 1
+    ac, ai = _arr_of(d, 0, n3), _arr_of(d, 1, n3 + 1)
+    L1, L2 = m.domain.hsize
+    k1 = np.arange(K1)
+    k2 = np.concatenate([np.arange(m.k2max + 1), np.arange(-m.k2max, 0)])
+    D1 = (1j * 2 * np.pi * k1 / L1)[:, None, None]
+    D2 = (1j * 2 * np.pi * k2 / L2)[None, :, None]
+    u, v, w = (state[f] + dt * rates[f] for f in ("vel1", "vel2", "vel3"))
+    z = np.zeros((K1, K2, 1), complex)
+    rhs = (D1 * u + D2 * v) / ac[None, None, :] + np.concatenate([w, z], axis=2) - np.concatenate([z, w], axis=2)
+    rhs[0, 0, -1] = 0  # walls at w = 0; the (0,0) row pinned
+    d3g3 = np.zeros(n3)
+    d3g3[1:] -= ai[1:n3]
+    d3g3[:-1] -= ai[1:n3]
+    kk1, kk2 = np.tile(k1, K2), np.repeat(k2, K1)
+    shift = 4 * np.pi ** 2 * (kk1 ** 2 / L1 ** 2 + kk2 ** 2 / L2 ** 2)
+    dvs = d3g3[None, :] - shift[:, None] / ac[None, :]
+    dvs[0] = d3g3
+    dvs[0, -1] = -3 * ai[n3 - 1]
+    evs = np.broadcast_to(ai[1:n3], (K1 * K2, n3 - 1)).copy()
+    b = rhs.reshape((K1 * K2, n3), order="F")
+    return B.BatchLDLt(dvs, evs, precision="fp64").solve(b)
+
+
+def form_error(B, res, dt, checkpoints=(100, 500)):
+    """Error of the chosen (mixed) form against the plain float64 form on the
+    GPU, both from the same projected state (log law + seeded noise): per
+    field the relative L2 and max-norm difference after each checkpoint's
+    number of SSPRK33 steps, plus that of the pressure the next stage solves
+    for.  Relative to the float64 field's L2 norm / maximum."""
+    m64 = make_les(B, "fp64", res)
+    st0 = m64.state
+    m32 = B.Model(*B.workloads.les_neutral(B, res), precision="fp32")
+    m32.set_state(**st0)
+    m32.init_problem(0.0)
+    out, done = {}, 0
+    for n in checkpoints:
+        m64.step(dt, n - done)
+        m32.step(dt, n - done)
+        done = n
+        a, b = m32.state, m64.state
+        e = {}
+        for f in ("vel1", "vel2", "vel3"):
+            d = a[f] - b[f]
+            e[f] = dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(b[f])),
+                        rel_max=float(np.abs(d).max() / np.abs(b[f]).max()))
+        p32 = pressure_of(B, m32, a, m32.rates(), dt)
+        p64 = pressure_of(B, m64, b, m64.rates(), dt)
+        d = p32 - p64
+        e["pressure"] = dict(rel_l2=float(np.linalg.norm(d) / np.linalg.norm(p64)),
+                             rel_max=float(np.abs(d).max() / np.abs(p64).max()))
+        out[f"steps_{n}"] = {f: {k: float(f"{x:.3e}") for k, x in v.items()} for f, v in e.items()}
+    return out
 
 
 def pressure_solve(B, nz=256, n=256, reps=5):
@@ -444,6 +613,7 @@ def main():
     ap.add_argument("--no-pressure", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-scale copy kernel")
     ap.add_argument("--rocprof-summary", nargs=5, metavar="DIR",
@@ -482,6 +652,16 @@ def main():
                     kernels={k: dict(avg_us=round(v["avg_us"], 2), gbs=round(v["gbs"], 1)) for k, v in per64.items()})
         del m64
         log("fp64:", r64)
+    c1 = None
+    if not args.no_c1:
+        log("C1 64^3 DNS fp64 ...")
+        c1 = c1_bench(B, cpu_seconds=0.0 if args.no_cpu else 8.0)
+        log("c1:", c1)
+    ferr = None
+    if not args.no_fp64:
+        log("fp32 vs fp64 over 500 steps ...")
+        ferr = form_error(B, res, args.dt)
+        log("form error:", ferr)
     pres = None if args.no_pressure else pressure_solve(B)
     ens = None
     if not args.no_ensemble:
@@ -522,6 +702,8 @@ def main():
         "speedup_vs_fp64_plain": round(sps / fp64["steps_per_s"], 3) if fp64 else None,
         "kernels_fp32": {k: dict(avg_us=round(v["avg_us"], 2), launches=v["launches"], gbs=round(v["gbs"], 1))
                          for k, v in per32.items()},
+        "fp32_vs_fp64_500": ferr,
+        "dns_64cubed_fp64": c1,
         "roofline": roof,
         "fft_autotune": tuning32,
         "pressure_solve_65536x256": pres,

@@ -133,11 +133,12 @@ def r2c_fused(N1, N2, nk1, precision):
     return lds <= 160 * 1024 and nk1 * max(A, N // A) <= 1024
 
 
-def algorithmic_bytes(cfg_sizes, kind, precision):
+def algorithmic_bytes(cfg_sizes, kind, precision, path=None):
     """Minimal HBM bytes of each kernel class per right-hand side over all
     spaces (stage_project: per stage; inputs read once, outputs written once).
     ``cfg_sizes`` holds K1, K2, n3 and the physical sizes of the advection and
-    SGS spaces."""
+    SGS spaces; ``path`` the transform path in use ({"c2r": ..., "r2c": ...}
+    as `Model.tuning()` reports it; None: whole-plane wherever a space fits)."""
     bc = 8 if precision == "fp32" else 16
     br = bc // 2
     K1, K2, n3 = cfg_sizes["K1"], cfg_sizes["K2"], cfg_sizes["n3"]
@@ -151,14 +152,16 @@ def algorithmic_bytes(cfg_sizes, kind, precision):
 
     terms = {"adv": (3 * nC + 3 * nI), "sgs": (6 * nC + 3 * nI)}
     srcs = {"adv": (2 * nC + nI), "sgs": (5 * nC + 3 * nI)}
-    fused = [sp for sp in spaces if c2r_fused(sp[1], sp[2], nk1(sp[1]), precision)]
+    c2r_whole = path is None or path["c2r"] != "two-pass"
+    r2c_whole = path is None or path["r2c"] != "two-pass"
+    fused = [sp for sp in spaces if c2r_whole and c2r_fused(sp[1], sp[2], nk1(sp[1]), precision)]
     paired = [sp for sp in spaces if sp not in fused]
     out = {}
     out["linear"] = 2 * S
     out["c2r_2d"] = sum(terms[k] * (nk * bc + N1 * N2 * br) for k, N1, N2 in fused)
     out["c2r_x2"] = sum(terms[k] * (nk * bc + N2 * nk1(N1) * bc) for k, N1, N2 in paired)
     out["c2r_x1"] = sum(terms[k] * (N2 * nk1(N1) * bc + N1 * N2 * br) for k, N1, N2 in paired)
-    fused_r = [sp for sp in spaces if r2c_fused(sp[1], sp[2], nk1(sp[1]), precision)]
+    fused_r = [sp for sp in spaces if r2c_whole and r2c_fused(sp[1], sp[2], nk1(sp[1]), precision)]
     paired_r = [sp for sp in spaces if sp not in fused_r]
     out["r2c_x1"] = sum(srcs[k] * (N1 * N2 * br + N2 * nk1(N1) * bc) for k, N1, N2 in paired_r)
     # fused r2c: every source plane of a target read (T12 serves two targets),
