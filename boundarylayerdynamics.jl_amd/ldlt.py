@@ -42,7 +42,9 @@ class BatchLDLt:
 
     def solve(self, b):
         """x = A \\ b for b of shape (nk, nz), complex."""
-        buf = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).T)  # [nz][nk]
+        # a copy in [nz][nk] order (the solve works in place; b.T can already be
+        # contiguous, for one level or one system, and must not be overwritten)
+        buf = np.array(np.asarray(b, dtype=np.complex128).T, order="C", copy=True)
         self._check(self._lib.bld_ldlt_solve(self._h, dptr(buf)))
         return buf.T.copy()
 

@@ -309,6 +309,19 @@ def test_ldlt_dense(N):
     assert relerr(x, xo) < 1e-14
 
 
+@pytest.mark.parametrize("nk,nz", [(10, 1), (1, 12)])
+def test_ldlt_solve_keeps_its_input(nk, nz):
+    """One level or one system: the transposed right-hand side is contiguous
+    already, and the in-place device solve must still not overwrite it."""
+    rng = np.random.default_rng(nk + nz)
+    dv, ev = -(rng.random((nk, nz)) + 2.5), rng.random((nk, nz - 1))
+    b = rng.standard_normal((nk, nz)) + 1j * rng.standard_normal((nk, nz))
+    b0 = b.copy()
+    x = B.BatchLDLt(dv, ev).solve(b)
+    assert np.array_equal(b, b0)
+    assert relerr(x, O.BatchLDLt(dv, ev).solve(b0.copy())) < 1e-14
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
 @pytest.mark.parametrize("nz", [17, 300])
 def test_ldlt_tall_systems(precision, tol, nz):
