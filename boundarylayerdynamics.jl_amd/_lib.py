@@ -35,7 +35,7 @@ EXPORTS = ("bld_create", "bld_destroy", "bld_ensemble_create", "bld_ensemble_siz
            "bld_mlp_nparams", "bld_mlp_create", "bld_mlp_destroy", "bld_mlp_last_error", "bld_mlp_get_params",
            "bld_mlp_forward", "bld_mlp_gradient", "bld_mlp_set_data", "bld_mlp_train", "bld_mlp_time",
            "bld_sgs_closure", "bld_closure_sample", "bld_set_method", "bld_profiles_enable", "bld_profiles_read",
-           "bld_set_transform_path")
+           "bld_set_transform_path", "bld_ldlt_solve_noalloc", "bld_ldlt_noalloc_last_error")
 
 
 class BldBC(ctypes.Structure):
@@ -88,6 +88,8 @@ def load():
         "bld_autotune": ([P, ctypes.c_int], ctypes.c_int),
         "bld_tuning": ([P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), D], ctypes.c_int),
         "bld_set_transform_path": ([P, ctypes.c_int], ctypes.c_int),
+        "bld_ldlt_solve_noalloc": ([i64, i64, P, P, P, ctypes.c_int, P], ctypes.c_int),
+        "bld_ldlt_noalloc_last_error": ([], ctypes.c_char_p),
         "bld_last_error": ([P], ctypes.c_char_p),
         "bld_set_state": ([P, D, D, D], ctypes.c_int),
         "bld_get_state": ([P, D, D, D], ctypes.c_int),

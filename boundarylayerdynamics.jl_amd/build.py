@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 KERNEL_PARTS = 9  # kernels.hip: part 0 (physics, stages, launchers) and eight transform parts
 UNITS = ([("csrc/kernels.hip", [f"-DBLD_KERNEL_PART={p}"]) for p in range(KERNEL_PARTS)]
-         + [(s, []) for s in ("csrc/engine.hip", "csrc/mlp.hip", "csrc/closure.hip", "csrc/profiles.hip")])
+         + [(s, []) for s in ("csrc/engine.hip", "csrc/mlp.hip", "csrc/closure.hip", "csrc/profiles.hip",
+                              "csrc/noalloc.hip")])
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -13,6 +13,7 @@
 
 #include "bld.h"
 #include "kernels.hpp"
+#include "ldlt.hpp"
 #include "mlp.hpp"
 #include "transforms_2d.hpp"
 
@@ -2048,6 +2049,28 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
     }
 }
 
+// One chunked LDLt solve (k_ldlt_chunked: NC chunks of 16 levels per system,
+// MB = 256/NC systems per block) of nk systems of nz levels on stream st.
+void ldlt_chunked_launch(bool fp32_ir, const void *gamma, const double *beta, const double *dv, const void *b, void *x,
+                         int nk, int nz, hipStream_t st) {
+    constexpr int CL = 16;
+    int NC = 1;
+    while (NC * CL < nz) NC *= 2;
+    const int MB = 256 / NC;
+    const int nbk = (nk + MB - 1) / MB;
+    if (!fp32_ir) {
+        const size_t sm = 256 * (sizeof(cplx<double>) + sizeof(double));
+        k_ldlt_chunked<double, CL, false><<<nbk, 256, sm, st>>>(static_cast<const double *>(gamma), beta, nullptr,
+                                                                 static_cast<const cplx<double> *>(b),
+                                                                 static_cast<cplx<double> *>(x), nk, nz, MB);
+    } else {
+        const size_t sm = 256 * (3 * sizeof(cplx<float>) + sizeof(float));
+        k_ldlt_chunked<float, CL, true><<<nbk, 256, sm, st>>>(static_cast<const float *>(gamma), beta, dv,
+                                                               static_cast<const cplx<float> *>(b),
+                                                               static_cast<cplx<float> *>(x), nk, nz, MB);
+    }
+}
+
 struct Ldlt {
     int64_t nk = 0, nz = 0;
     int precision = 0;
@@ -2375,23 +2398,12 @@ void bld_ldlt_destroy(bld_ldlt *s) { delete s; }
 const char *bld_ldlt_last_error(const bld_ldlt *s) { return s ? s->s.err.c_str() : g_create_error.c_str(); }
 
 static void ldlt_run(Ldlt &s, hipStream_t st) {
-    constexpr int CL = 16;
-    int NC = 1;
-    while (NC * CL < s.nz) NC *= 2;
-    if (NC <= 256) {  // chunked: NC chunks of CL levels per system, MB systems per block
-        const int MB = 256 / NC;
-        const int nbk = int((s.nk + MB - 1) / MB);
-        if (s.precision == BLD_FP64) {
-            const size_t sm = 256 * (sizeof(cplx<double>) + sizeof(double));
-            k_ldlt_chunked<double, CL, false><<<nbk, 256, sm, st>>>(s.gamma.as<double>(), s.beta.as<double>(), nullptr,
-                                                                     s.b64.as<cplx<double>>(), s.b64.as<cplx<double>>(),
-                                                                     int(s.nk), int(s.nz), MB);
-        } else {
-            const size_t sm = 256 * (3 * sizeof(cplx<float>) + sizeof(float));
-            k_ldlt_chunked<float, CL, true><<<nbk, 256, sm, st>>>(s.gamma32.as<float>(), s.beta.as<double>(),
-                                                                   s.dv.as<double>(), s.b32.as<cplx<float>>(),
-                                                                   s.x32.as<cplx<float>>(), int(s.nk), int(s.nz), MB);
-        }
+    if (s.nz <= kLdltChunkedMaxLevels) {
+        if (s.precision == BLD_FP64)
+            ldlt_chunked_launch(false, s.gamma.p, s.beta.as<double>(), nullptr, s.b64.p, s.b64.p, int(s.nk), int(s.nz), st);
+        else
+            ldlt_chunked_launch(true, s.gamma32.p, s.beta.as<double>(), s.dv.as<double>(), s.b32.p, s.x32.p, int(s.nk),
+                                int(s.nz), st);
         return;
     }
     const int nb = int((s.nk + 63) / 64);  // very tall systems: one thread per system

@@ -255,6 +255,25 @@ int bld_ldlt_solve(bld_ldlt *s, double *b);
 int bld_ldlt_bench(bld_ldlt *s, const double *b, int reps, double *ms_per_solve, double *x_out);
 const char *bld_ldlt_last_error(const bld_ldlt *s);
 
+/* Allocation-free variant of bld_ldlt_create + bld_ldlt_solve.  No reference
+ * counterpart: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops).  dvs [nz][nk], evs [nz-1][nk] and the complex
+ * right-hand side b [nz][nk] (interleaved, overwritten with the solution) are
+ * DEVICE pointers; the kernels run on `stream` (a hipStream_t, NULL for the
+ * null stream) and the call returns without waiting for them.  The factor and
+ * the float32 staging live in a fixed device array of
+ * BLD_LDLT_NOALLOC_SCRATCH bytes that every GPU the library runs on holds
+ * until the process ends: float64 needs 8 B per unknown (8,388,608 unknowns
+ * at most), BLD_FP32_MIXED 20 B (3,355,443 at most); systems have at most
+ * 4096 levels.  Larger sizes return BLD_EARG with a message.  Every call on
+ * a GPU shares that GPU's array (each call overwrites all of the part it
+ * uses, nothing needs clearing), so calls that overlap on one GPU race.  The
+ * message of the last failed call on this thread: bld_ldlt_noalloc_last_error. */
+#define BLD_LDLT_NOALLOC_SCRATCH (64 << 20)
+int bld_ldlt_solve_noalloc(int64_t nk, int64_t nz, const double *dvs, const double *evs, double *b, int precision,
+                           void *stream);
+const char *bld_ldlt_noalloc_last_error(void);
+
 /* Learned SGS closure network (no reference counterpart; it sits behind the
  * process API next to StaticSmagorinskyModel, static_smagorinsky.jl:39-42):
  * 10 features (padded to 16) → 64 → 64 (GELU) → 6 outputs (padded to 16),
