@@ -95,15 +95,24 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     cplx<T> *Z = smem;
     cplx<T> *TW = smem + size_t(N) * zs;
     const size_t P2 = size_t(K1) * K2;
-    // the twiddle table, loaded together with the first stage's data
-    for (int j = threadIdx.x; j < N; j += NT) TW[j] = gload(S.tw + S.p2.tw + j);
+    // the twiddle table, loaded together with the first stage's data and
+    // stored to LDS behind the first round's gathers
+    cplx<T> twv[(N + NT - 1) / NT];
+    static_for<0, (N + NT - 1) / NT>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int e = j * NT + int(threadIdx.x);
+        twv[j] = e < N ? gload(S.tw + S.p2.tw + e) : czero<T>();
+    });
     const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
 
-    // x2, stage 1: item (c, t), c fastest; A values at k2 positions Bq·n + t
+    // x2, stage 1: item (c, t), c fastest; A values at k2 positions Bq·n + t,
+    // in rounds of NT items (the same count for every thread)
     {
-        const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;
-        for (int it = threadIdx.x; it < nk1 * Bq; it += NT) {
-            const int t = it / nk1, c = it - t * nk1;
+        const int items = nk1 * Bq, rounds = (items + NT - 1) / NT;
+        for (int rd = 0; rd < rounds; ++rd) {
+            const int it = rd * NT + int(threadIdx.x);
+            const bool have = it < items;
+            const int t = (have ? it : 0) / nk1, c = (have ? it : 0) - t * nk1;
             int i2s[A];
             bool ok[A];
             static_for<0, A>([&](auto n1c) {
@@ -113,33 +122,66 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
                 ok[n1] = lo || hi;
                 i2s[n1] = lo ? pos : (hi ? K2 - (N - pos) : 0);
             });
-            const size_t base = size_t(layer) * P2 + c;
+            const uint32_t base = uint32_t(layer) * uint32_t(P2) + uint32_t(c);
             cplx<T> v[A];
+            // padding positions (k2 between the resolved halves) load nothing:
+            // with 64 columns per wave their test is uniform across the wave
             auto gather = [&](auto value) {
                 static_for<0, A>([&](auto n1c) {
                     constexpr int n1 = decltype(n1c)::value;
-                    const cplx<T> x = value(base + size_t(K1) * i2s[n1], i2s[n1]);
-                    v[n1] = ok[n1] ? x : czero<T>();
+                    v[n1] = czero<T>();
+                    if (have && ok[n1]) v[n1] = value(base + uint32_t(K1) * uint32_t(i2s[n1]), i2s[n1]);
                 });
             };
             const T d1 = T(gload(S.D1 + c));
+            auto d2 = [&](int i2) { return T(gload(S.D2 + i2)); };
             switch (recipe) {  // uniform per block
-                case R_U: gather([&](size_t i, int) { return gload(u + i); }); break;
-                case R_V: gather([&](size_t i, int) { return gload(vv + i); }); break;
-                case R_W: gather([&](size_t i, int) { return gload(w + i); }); break;
-                case R_D1U: gather([&](size_t i, int) { return imul(d1, gload(u + i)); }); break;
-                case R_D1V: gather([&](size_t i, int) { return imul(d1, gload(vv + i)); }); break;
-                case R_D1W: gather([&](size_t i, int) { return imul(d1, gload(w + i)); }); break;
-                case R_D2U: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(u + i)); }); break;
-                case R_D2V: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(vv + i)); }); break;
-                case R_D2W: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(w + i)); }); break;
-                default: gather([&](size_t i, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
+                case R_U: gather([&](uint32_t i, int) { return gload(u + i); }); break;
+                case R_V: gather([&](uint32_t i, int) { return gload(vv + i); }); break;
+                case R_W: gather([&](uint32_t i, int) { return gload(w + i); }); break;
+                case R_D1U: gather([&](uint32_t i, int) { return imul(d1, gload(u + i)); }); break;
+                case R_D1V: gather([&](uint32_t i, int) { return imul(d1, gload(vv + i)); }); break;
+                case R_D1W: gather([&](uint32_t i, int) { return imul(d1, gload(w + i)); }); break;
+                case R_D2U: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(u + i)); }); break;
+                case R_D2V: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(vv + i)); }); break;
+                case R_D2W: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(w + i)); }); break;
+                // vorticity (derivatives.jl:566-568), the operations of term_value
+                case R_VORT1: {
+                    const T a = T(gload(S.alpha_i + layer + 1));
+                    gather([&](uint32_t i, int i2) {
+                        const cplx<T> dz = (-a) * gload(vv + i) + a * gload(vv + i + P2);
+                        return imul(d2(i2), gload(w + i)) - dz;
+                    });
+                    break;
+                }
+                case R_VORT2: {
+                    const T a = T(gload(S.alpha_i + layer + 1));
+                    gather([&](uint32_t i, int) {
+                        const cplx<T> dz = (-a) * gload(u + i) + a * gload(u + i + P2);
+                        return dz - imul(d1, gload(w + i));
+                    });
+                    break;
+                }
+                case R_VORT3:
+                    gather([&](uint32_t i, int i2) { return imul(d1, gload(vv + i)) - imul(d2(i2), gload(u + i)); });
+                    break;
+                default: gather([&](uint32_t, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
             }
-            rdft<T, true, A>(v);
-            static_for<0, A>([&](auto kc) {
-                constexpr int k = decltype(kc)::value;
-                Z[(t * A + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw, t * k));
-            });
+            if (rd == 0) {  // the twiddles in LDS for every round
+                static_for<0, (N + NT - 1) / NT>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    const int e = j * NT + int(threadIdx.x);
+                    if (e < N) TW[e] = twv[j];
+                });
+                __syncthreads();
+            }
+            if (have) {
+                rdft<T, true, A>(v);
+                static_for<0, A>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    Z[(t * A + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, t * k));
+                });
+            }
         }
     }
     __syncthreads();
@@ -383,12 +425,21 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     cplx<T> *X = smem + size_t(N) * zs;
     cplx<T> *TW = X + P * PS;
     cplx<T> *DF = TW + N;  // D1[k1] in .x
-    const cplx<T> *__restrict__ tw1 = S.tw + S.p1.tw;
     const int tid = threadIdx.x;
     // the twiddle table for the x2 stages and the D1 factors of the row split
     // (visible after the first barrier)
-    for (int j = tid; j < N; j += blockDim.x) TW[j] = gload(S.tw + S.p2.tw + j);
-    for (int j = tid; j < nk1; j += blockDim.x) DF[j] = mk(T(gload(S.D1 + j)), T(0));
+    // (both stored to LDS behind the first round's row loads: square grids
+    // share one twiddle table between the directions)
+    constexpr int NTW = (N + kC2r2dThreads - 1) / kC2r2dThreads;
+    cplx<T> twv[NTW];
+    static_for<0, NTW>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int e = j * kC2r2dThreads + tid;
+        twv[j] = e < N ? gload(S.tw + S.p2.tw + e) : czero<T>();
+    });
+    const T dfv = tid < nk1 ? T(gload(S.D1 + tid)) : T(0);
+    static_assert(P * Bq <= kC2r2dThreads, "one row item per thread and round");
+    bool tw_ready = false;  // block-uniform
     const T scale = T(1.0 / (double(N) * double(N)));
     cplx<T> *out = S.tgt_out[tg] + size_t(layer) * K1 * K2;
 
@@ -397,20 +448,33 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
         const T *in = S.src_phys[src] + size_t(layer) * N * N;
         for (int p0 = 0; p0 < NP; p0 += P) {
             // stage 1: item (pair, t), A values of both rows at x = Bq·n + t
-            for (int it = tid; it < P * Bq; it += blockDim.x) {
-                const int pl = it / Bq, t = it - pl * Bq;
+            {
+                const bool have = tid < P * Bq;
+                const int pl = (have ? tid : 0) / Bq, t = (have ? tid : 0) - pl * Bq;
                 const int p = min(p0 + pl, NP - 1);
                 const T *ra = in + size_t(2 * p) * N, *rb = ra + N;
                 cplx<T> v[A];
                 static_for<0, A>([&](auto nc) {
                     constexpr int n = decltype(nc)::value;
-                    v[n] = mk(gload(ra + Bq * n + t), gload(rb + Bq * n + t));
+                    v[n] = have ? mk(gload(ra + Bq * n + t), gload(rb + Bq * n + t)) : czero<T>();
                 });
-                rdft<T, false, A>(v);
-                static_for<0, A>([&](auto kc) {
-                    constexpr int k = decltype(kc)::value;
-                    X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, false>(tw1, t * k));
-                });
+                if (!tw_ready) {
+                    static_for<0, NTW>([&](auto jc) {
+                        constexpr int j = decltype(jc)::value;
+                        const int e = j * kC2r2dThreads + tid;
+                        if (e < N) TW[e] = twv[j];
+                    });
+                    if (tid < nk1) DF[tid] = mk(dfv, T(0));
+                    __syncthreads();
+                    tw_ready = true;
+                }
+                if (have) {
+                    rdft<T, false, A>(v);
+                    static_for<0, A>([&](auto kc) {
+                        constexpr int k = decltype(kc)::value;
+                        X[pl * PS + t * RS + k] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, false>(TW, t * k));
+                    });
+                }
             }
             __syncthreads();
             // stage 2 in place: spectrum index t + A·k of the pair at X[pl][k·RS + t]
