@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-KERNEL_PARTS = 9  # kernels.hip: part 0 (physics, stages, launchers) and eight transform parts
+KERNEL_PARTS = 17  # kernels.hip: part 0 (physics, stages, launchers) and one part per transform kind and type
 UNITS = ([("csrc/kernels.hip", [f"-DBLD_KERNEL_PART={p}"]) for p in range(KERNEL_PARTS)]
          + [(s, []) for s in ("csrc/engine.hip", "csrc/mlp.hip", "csrc/closure.hip", "csrc/profiles.hip",
                               "csrc/noalloc.hip")])

@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(kC2r2dThreads) k_r2c_2d(const Space<T> *__rest
     const Space<T> &S = spaces[E.space];
 #define BLD_CALL(n)                       \
     if constexpr (c2r2d_supported<T>(n)) \
-        r2c_2d_reg<T, n>(S, E, smem)
+        r2c_2d_any<T, n>(S, E, smem)
     BLD_FFT_SWITCH(S.N2, BLD_CALL)
 #undef BLD_CALL
 }
@@ -1266,22 +1266,11 @@ void launch_fill(void *a, size_t n, hipStream_t st) {
 BLD_INSTANTIATE(float)
 BLD_INSTANTIATE(double)
 
-#elif BLD_KERNEL_PART == 1
-BLD_XF_INST(float, PASS_C2R_X2) BLD_XF_INST(float, PASS_C2R_X1)
-#elif BLD_KERNEL_PART == 2
-BLD_XF_INST(float, PASS_R2C_X1) BLD_XF_INST(float, PASS_R2C_X2)
-#elif BLD_KERNEL_PART == 3
-BLD_XF_INST(float, PASS_C2R_2D) BLD_XF_INST(float, PASS_C2R_2D_HALF)
-#elif BLD_KERNEL_PART == 4
-BLD_XF_INST(float, PASS_C2R_2H) BLD_XF_INST(float, PASS_R2C_2D)
-#elif BLD_KERNEL_PART == 5
-BLD_XF_INST(double, PASS_C2R_X2) BLD_XF_INST(double, PASS_C2R_X1)
-#elif BLD_KERNEL_PART == 6
-BLD_XF_INST(double, PASS_R2C_X1) BLD_XF_INST(double, PASS_R2C_X2)
-#elif BLD_KERNEL_PART == 7
-BLD_XF_INST(double, PASS_C2R_2D) BLD_XF_INST(double, PASS_C2R_2D_HALF)
-#elif BLD_KERNEL_PART == 8
-BLD_XF_INST(double, PASS_C2R_2H) BLD_XF_INST(double, PASS_R2C_2D)
+#else
+// parts 1..16: one transform kind of one type each (kind = (part − 1) % 8 in
+// PassKind order, float for parts 1..8, double for 9..16)
+#define BLD_PART_T std::conditional_t<(BLD_KERNEL_PART <= 8), float, double>
+BLD_XF_INST(BLD_PART_T, ((BLD_KERNEL_PART - 1) % 8))
 #endif
 
 }  // namespace bld

@@ -81,7 +81,21 @@ __host__ __device__ constexpr bool c2r2h_supported(int N) {
 }
 
 
-template <typename T, int N, int NT>
+// Spectral layouts with compile-time sizes: KM = k1max = k2max with every
+// resolved mode inside the plane (K1 = nk1 = KM+1, K2 = 2·KM+1), the layouts
+// of square grids whose physical size is N (default size N = 2·KM+2, or the
+// dealiased 3·KM+3).  KM < 0: sizes read from the space descriptor.
+__host__ __device__ constexpr bool xf2d_km_special(int N, int KM) {
+    return (N == 192 && KM == 63) || (N == 128 && KM == 63) || (N == 144 && KM == 47) ||
+           (N == 96 && (KM == 31 || KM == 47)) || (N == 72 && KM == 23) || (N == 64 && KM == 31) ||
+           (N == 48 && KM == 23);
+}
+template <typename T>
+__device__ __forceinline__ bool xf2d_layout_is(const Space<T> &S, int KM) {
+    return S.k1m == KM && S.k2m == KM && S.nk1 == KM + 1 && S.K1 == KM + 1 && S.K2 == 2 * KM + 1;
+}
+
+template <typename T, int N, int NT, int KM = -1>
 __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry &E, int sel, cplx<T> *smem) {
     constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1;
     constexpr int P = c2r2d_pairs(N, NT), NP = N / 2;
@@ -91,7 +105,9 @@ __device__ __forceinline__ void c2r_2d_reg(const Space<T> &S, const LaunchEntry 
     while (job >= S.in_job0[term + 1]) ++term;
     const int layer = job - S.in_job0[term];
     const int recipe = S.in_recipe[term];
-    const int nk1 = S.nk1, zs = c2r2d_zs(N, nk1), k1m = S.k1m, k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    const int nk1 = KM >= 0 ? KM + 1 : S.nk1, zs = c2r2d_zs(N, nk1);
+    const int k1m = KM >= 0 ? KM : S.k1m, k2m = KM >= 0 ? KM : S.k2m;
+    const int K1 = KM >= 0 ? KM + 1 : S.K1, K2 = KM >= 0 ? 2 * KM + 1 : S.K2;
     cplx<T> *Z = smem;
     cplx<T> *TW = smem + size_t(N) * zs;
     const size_t P2 = size_t(K1) * K2;
@@ -411,7 +427,7 @@ __device__ __forceinline__ void c2r_2h_reg(const Space<T> &S, const LaunchEntry 
 // to the target.  Sources that take i·D2 after the transform form a second
 // sum, transformed the same way and added afterwards.  Neither the row spectra
 // nor the sums go to HBM.
-template <typename T, int N>
+template <typename T, int N, int KM = -1>
 __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
     constexpr int A = reg_fft_a(N), Bq = N / A, RS = A + 1, PS = Bq * RS;
     constexpr int P = r2c2d_pairs(N), NP = N / 2;
@@ -419,7 +435,8 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     int tg = 0;
     while (job >= S.tgt_job0[tg + 1]) ++tg;
     const int layer = job - S.tgt_job0[tg];
-    const int nk1 = S.nk1, zs = r2c2d_zs(N, nk1), k2m = S.k2m, K1 = S.K1, K2 = S.K2;
+    const int nk1 = KM >= 0 ? KM + 1 : S.nk1, zs = r2c2d_zs(N, nk1), k2m = KM >= 0 ? KM : S.k2m;
+    const int K1 = KM >= 0 ? KM + 1 : S.K1, K2 = KM >= 0 ? 2 * KM + 1 : S.K2;
     const int nG = S.tgt_nG[tg], nH = S.tgt_nH[tg];
     cplx<T> *Z = smem;
     cplx<T> *X = smem + size_t(N) * zs;
@@ -572,6 +589,20 @@ __device__ __forceinline__ void r2c_2d_reg(const Space<T> &S, const LaunchEntry 
     if (nG > 0) columns(false);
     for (int s = 0; s < nH; ++s) rows(S.tgt_H[tg][s], false, s == 0);
     if (nH > 0) columns(true);
+}
+
+// The whole-plane r2c for length N: the compile-time layout where the space
+// has one (the c2r keeps the descriptor's sizes: compile-time sizes unroll
+// its loops beyond the register budget of its 1024-thread blocks).
+template <typename T, int N>
+__device__ __forceinline__ void r2c_2d_any(const Space<T> &S, const LaunchEntry &E, cplx<T> *smem) {
+#define BLD_KM(km)                                                        \
+    if constexpr (xf2d_km_special(N, km)) {                               \
+        if (xf2d_layout_is(S, km)) return r2c_2d_reg<T, N, km>(S, E, smem); \
+    }
+    BLD_KM(23) BLD_KM(31) BLD_KM(47) BLD_KM(63)
+#undef BLD_KM
+    r2c_2d_reg<T, N>(S, E, smem);
 }
 
 }  // namespace bld
