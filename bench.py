@@ -616,6 +616,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--autotune", type=int, default=1)
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-scale copy kernel")
+    ap.add_argument("--pressure-only", action="store_true",
+                    help="only the isolated pressure solve (65,536 systems x 256 levels), for profiling")
     ap.add_argument("--rocprof-summary", nargs=5, metavar="DIR",
                     help="summarise rocprofv3 output: kernel-trace, FETCH_SIZE, WRITE_SIZE, copy FETCH, copy WRITE")
     args = ap.parse_args()
@@ -628,6 +630,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.pressure_only:
+        print(json.dumps({"pressure_solve_65536x256": pressure_solve(load_package(), reps=20)}), flush=True)
         return
     res = tuple(int(x) for x in args.res.split(","))
     B = load_package()

@@ -1924,7 +1924,8 @@ __global__ void __launch_bounds__(64) k_ldlt32_ir(const float *__restrict__ gamm
 template <typename A, int CL, bool IR>
 __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamma, const double *__restrict__ beta,
                                                       const double *__restrict__ dv, const cplx<A> *b,
-                                                      cplx<A> *x, int nk, int nz, int MB) {  // b may be x
+                                                      cplx<A> *x, int nk, int nz, int MB,
+                                                      int beta_per_mode) {  // b may be x
     extern __shared__ unsigned char smem_raw[];
     const int NC = blockDim.x / MB;
     cplx<A> *s_y = reinterpret_cast<cplx<A> *>(smem_raw);  // [NC][MB]
@@ -1935,6 +1936,9 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
     const bool active = m < nk;
     const int mm = active ? m : 0;
     const int l0 = ty * CL;
+    // β of level l: beta[l][m] ([nz-1][nk]) or, for a mode-independent
+    // off-diagonal, beta[l]
+    const size_t bls = beta_per_mode ? size_t(nk) : 1, bmo = beta_per_mode ? size_t(mm) : 0;
     A g[CL], bl[CL], bu[CL];
     cplx<A> r[CL];
 #pragma unroll
@@ -1942,8 +1946,8 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
         const int lc = min(l0 + j, nz - 1);
         g[j] = gload(gamma + size_t(lc) * nk + mm);
         r[j] = gload(b + size_t(lc) * nk + mm);
-        bl[j] = A(gload(beta + size_t(max(lc - 1, 0)) * nk + mm));  // couples l-1 and l
-        bu[j] = A(gload(beta + size_t(min(lc, nz - 2 < 0 ? 0 : nz - 2)) * nk + mm));  // couples l and l+1
+        bl[j] = A(gload(beta + size_t(max(lc - 1, 0)) * bls + bmo));  // couples l-1 and l
+        bu[j] = A(gload(beta + size_t(min(lc, nz - 2 < 0 ? 0 : nz - 2)) * bls + bmo));  // couples l and l+1
     }
     auto solve = [&](cplx<A>(&v)[CL]) {
         // forward sweep from a zero incoming value: y_l = g_l·(v_l − β_{l−1}·y_{l−1})
@@ -2032,8 +2036,8 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
             const cplx<A> xp = j + 1 < CL ? xs[j + 1] : above;
             // levels past nz (the last chunk's padding) load at a clamped level
             // and are discarded like every other value outside the column
-            const double blo = l > 0 ? double(gload(beta + size_t(min(max(l - 1, 0), max(nz - 2, 0))) * nk + mm)) : 0.0;
-            const double bup = l + 1 < nz ? double(gload(beta + size_t(min(l, max(nz - 2, 0))) * nk + mm)) : 0.0;
+            const double blo = l > 0 ? double(gload(beta + size_t(min(max(l - 1, 0), max(nz - 2, 0))) * bls + bmo)) : 0.0;
+            const double bup = l + 1 < nz ? double(gload(beta + size_t(min(l, max(nz - 2, 0))) * bls + bmo)) : 0.0;
             const double rx = double(r[j].x) - (d[j] * double(xs[j].x) + blo * double(xm.x) + (l + 1 < nz ? bup * double(xp.x) : 0.0));
             const double ry = double(r[j].y) - (d[j] * double(xs[j].y) + blo * double(xm.y) + (l + 1 < nz ? bup * double(xp.y) : 0.0));
             rs[j] = mk(A(rx), A(ry));
@@ -2051,8 +2055,8 @@ __global__ void __launch_bounds__(256) k_ldlt_chunked(const A *__restrict__ gamm
 
 // One chunked LDLt solve (k_ldlt_chunked: NC chunks of 16 levels per system,
 // MB = 256/NC systems per block) of nk systems of nz levels on stream st.
-void ldlt_chunked_launch(bool fp32_ir, const void *gamma, const double *beta, const double *dv, const void *b, void *x,
-                         int nk, int nz, hipStream_t st) {
+void ldlt_chunked_launch(bool fp32_ir, const void *gamma, const double *beta, bool beta_per_mode, const double *dv,
+                         const void *b, void *x, int nk, int nz, hipStream_t st) {
     constexpr int CL = 16;
     int NC = 1;
     while (NC * CL < nz) NC *= 2;
@@ -2062,18 +2066,21 @@ void ldlt_chunked_launch(bool fp32_ir, const void *gamma, const double *beta, co
         const size_t sm = 256 * (sizeof(cplx<double>) + sizeof(double));
         k_ldlt_chunked<double, CL, false><<<nbk, 256, sm, st>>>(static_cast<const double *>(gamma), beta, nullptr,
                                                                  static_cast<const cplx<double> *>(b),
-                                                                 static_cast<cplx<double> *>(x), nk, nz, MB);
+                                                                 static_cast<cplx<double> *>(x), nk, nz, MB,
+                                                                 beta_per_mode ? 1 : 0);
     } else {
         const size_t sm = 256 * (3 * sizeof(cplx<float>) + sizeof(float));
         k_ldlt_chunked<float, CL, true><<<nbk, 256, sm, st>>>(static_cast<const float *>(gamma), beta, dv,
                                                                static_cast<const cplx<float> *>(b),
-                                                               static_cast<cplx<float> *>(x), nk, nz, MB);
+                                                               static_cast<cplx<float> *>(x), nk, nz, MB,
+                                                               beta_per_mode ? 1 : 0);
     }
 }
 
 struct Ldlt {
     int64_t nk = 0, nz = 0;
     int precision = 0;
+    bool beta_per_level = false;  // off-diagonal the same for every system: stored per level
     DevBuf gamma, gamma32, beta, dv, b64, b32, x32, r32;
     std::string err;
 };
@@ -2375,7 +2382,24 @@ int bld_ldlt_create(int64_t nk, int64_t nz, const double *dvs, const double *evs
             }
         }
         upload(s.gamma, g);
-        upload(s.beta, beta);
+        // a mode-independent off-diagonal (the pressure operator's α at the
+        // I-nodes, pressure.jl:239-244) is kept as one value per level; the
+        // tall-system kernels read the full array
+        bool per_level = nz > 1 && nz <= kLdltChunkedMaxLevels;
+        for (int64_t i = 0; per_level && i + 1 < nz; ++i)
+            for (int64_t m = 1; m < nk; ++m)
+                if (evs[size_t(i) * nk + m] != evs[size_t(i) * nk]) {
+                    per_level = false;
+                    break;
+                }
+        s.beta_per_level = per_level;
+        if (per_level) {
+            std::vector<double> bl(nz, 0.0);
+            for (int64_t i = 0; i + 1 < nz; ++i) bl[i] = evs[size_t(i) * nk];
+            upload(s.beta, bl);
+        } else {
+            upload(s.beta, beta);
+        }
         if (precision == BLD_FP32_MIXED) {
             std::vector<float> g32(n);
             for (size_t i = 0; i < n; ++i) g32[i] = float(g[i]);
@@ -2400,10 +2424,11 @@ const char *bld_ldlt_last_error(const bld_ldlt *s) { return s ? s->s.err.c_str()
 static void ldlt_run(Ldlt &s, hipStream_t st) {
     if (s.nz <= kLdltChunkedMaxLevels) {
         if (s.precision == BLD_FP64)
-            ldlt_chunked_launch(false, s.gamma.p, s.beta.as<double>(), nullptr, s.b64.p, s.b64.p, int(s.nk), int(s.nz), st);
+            ldlt_chunked_launch(false, s.gamma.p, s.beta.as<double>(), !s.beta_per_level, nullptr, s.b64.p, s.b64.p,
+                                int(s.nk), int(s.nz), st);
         else
-            ldlt_chunked_launch(true, s.gamma32.p, s.beta.as<double>(), s.dv.as<double>(), s.b32.p, s.x32.p, int(s.nk),
-                                int(s.nz), st);
+            ldlt_chunked_launch(true, s.gamma32.p, s.beta.as<double>(), !s.beta_per_level, s.dv.as<double>(), s.b32.p,
+                                s.x32.p, int(s.nk), int(s.nz), st);
         return;
     }
     const int nb = int((s.nk + 63) / 64);  // very tall systems: one thread per system
@@ -2461,21 +2486,27 @@ int bld_ldlt_bench(bld_ldlt *h, const double *b, int reps, double *ms_per_solve,
         hipEvent_t e0, e1;
         HIPCHECK(hipEventCreate(&e0));
         HIPCHECK(hipEventCreate(&e1));
-        float total = 0;
-        for (int r = 0; r < reps; ++r) {  // the FP64 solve is in place: refresh b each repetition, untimed
-            ldlt_upload(s, b);
-            HIPCHECK(hipEventRecord(e0, nullptr));
-            ldlt_run(s, nullptr);
-            HIPCHECK(hipEventRecord(e1, nullptr));
-            HIPCHECK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-            total += ms;
-        }
+        // reps solves back to back between one pair of events (the float64
+        // solve works in place, so the timed solves after the first one solve
+        // for the previous solution: the same work on other values); then b
+        // again for the returned solution
+        ldlt_upload(s, b);
+        ldlt_run(s, nullptr);
+        HIPCHECK(hipEventRecord(e0, nullptr));
+        for (int r = 0; r < reps; ++r) ldlt_run(s, nullptr);
+        HIPCHECK(hipEventRecord(e1, nullptr));
+        HIPCHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        *ms_per_solve = total / reps;
-        if (x_out) ldlt_download(s, x_out);
+        *ms_per_solve = ms / reps;
+        if (x_out) {
+            ldlt_upload(s, b);
+            ldlt_run(s, nullptr);
+            HIPCHECK(hipDeviceSynchronize());
+            ldlt_download(s, x_out);
+        }
         return BLD_OK;
     });
 }

@@ -124,7 +124,7 @@ int bld_ldlt_solve_noalloc(int64_t nk, int64_t nz, const double *dvs, const doub
     if (precision == BLD_FP64) {
         double *gamma = static_cast<double *>(ptr);
         k_ldlt_factor<double><<<fb, 256, 0, st>>>(dvs, evs, gamma, int(nk), int(nz));
-        ldlt_chunked_launch(false, gamma, beta, nullptr, b, b, int(nk), int(nz), st);
+        ldlt_chunked_launch(false, gamma, beta, true, nullptr, b, b, int(nk), int(nz), st);
     } else {
         // [γ float n][b32 complex float n][x32 complex float n], each 256-B aligned
         auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -138,7 +138,7 @@ int bld_ldlt_solve_noalloc(int64_t nk, int64_t nz, const double *dvs, const doub
         }
         k_ldlt_factor<float><<<fb, 256, 0, st>>>(dvs, evs, gamma, int(nk), int(nz));
         k_to_f32<<<cb, 256, 0, st>>>(reinterpret_cast<const cplx<double> *>(b), b32, n);
-        ldlt_chunked_launch(true, gamma, beta, dvs, b32, x32, int(nk), int(nz), st);
+        ldlt_chunked_launch(true, gamma, beta, true, dvs, b32, x32, int(nk), int(nz), st);
         k_to_f64<<<cb, 256, 0, st>>>(x32, reinterpret_cast<cplx<double> *>(b), n);
     }
     e = hipGetLastError();

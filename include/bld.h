@@ -251,7 +251,9 @@ int bld_ldlt_create(int64_t nk, int64_t nz, const double *dvs, const double *evs
 void bld_ldlt_destroy(bld_ldlt *s);
 /* In-place solve of complex float64 b [nz][nk] (interleaved), host memory. */
 int bld_ldlt_solve(bld_ldlt *s, double *b);
-/* Device-resident solve: uploads b once, then times `reps` solves (ms each). */
+/* Device-resident timing: uploads b, then times `reps` solves run back to back
+ * between one pair of HIP events (ms per solve); x_out (optional) receives
+ * the solution of b. */
 int bld_ldlt_bench(bld_ldlt *s, const double *b, int reps, double *ms_per_solve, double *x_out);
 const char *bld_ldlt_last_error(const bld_ldlt *s);
 
