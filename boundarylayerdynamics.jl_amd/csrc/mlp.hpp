@@ -1,5 +1,7 @@
 // Learned SGS closure network: 16 (10 used) → 64 → 64 → 16 (6 used), GELU,
-// bfloat16 operands on MFMA (v_mfma_f32_16x16x16_bf16) with float32 accumulation.
+// bfloat16 operands on MFMA with float32 accumulation: the inference network
+// runs its hidden layers on v_mfma_f32_16x16x32_bf16 (gfx950), the first layer
+// (16 padded features) and the training kernels on v_mfma_f32_16x16x16_bf16.
 //
 // MFMA 16x16x16 operand layout (lane l, r = l % 16, g = l / 16, element j = 0..3):
 //   A[m][k]: lane holds A[r][4g + j]     B[k][n]: lane holds B[4g + j][r]
@@ -48,6 +50,23 @@ __device__ __forceinline__ float gelu_grad(float z) {
 __device__ __forceinline__ floatx4 mfma(bf16x4 a, bf16x4 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+
+// gfx950 v_mfma_f32_16x16x32_bf16: A[m][k] lane holds A[r][8g + j], B[k][n]
+// lane holds B[8g + j][r] (j = 0..7), D as above
+typedef short bf16x8 __attribute__((__vector_size__(8 * sizeof(short))));
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
+typedef float floatv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ floatx4 mfma32(bf16x8 a, bf16x8 b, floatx4 c) {
+    const floatv4 d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b),
+                                                              __builtin_bit_cast(floatv4, c), 0, 0, 0);
+    return __builtin_bit_cast(floatx4, d);
+}
+// Hidden-neuron order of the K = 32 blocks of the register-resident network:
+// element j of lane group g in K block c is neuron 16·(2c + j/4) + 4g + j%4,
+// i.e. the values the lane already holds in the D layouts of output blocks
+// 2c and 2c+1, so a layer's output feeds the next layer's 16x16x32 MFMA
+// without moving across lanes.
+__device__ __forceinline__ constexpr int k32_neuron(int c, int g, int j) { return 16 * (2 * c + j / 4) + 4 * g + j % 4; }
 
 // Forward pass of one 16-point tile by one wave.  `xa` is the A operand of the
 // input (lane holds x[point r][feature 4g+j] as bf16); weights in LDS (bf16,
@@ -152,7 +171,8 @@ __device__ __forceinline__ void forward_tile(const bf16x4 &xa, const unsigned sh
 // Z[point r][neuron 16t + 4g + j]) is exactly the B layout the next layer
 // needs, so activations never leave the registers.
 struct NetRegs {
-    bf16x4 w1[4], w2[4][4], w3[4];  // Wᵀ blocks: [out block][in block]
+    bf16x4 w1[4];                     // Wᵀ1 blocks [out block] (K = the 16 padded features)
+    bf16x8 w2[4][2], w3[2];           // Wᵀ2 [out block][K block of 32 neurons], Wᵀ3 [K block]
     float b1[4][4], b2[4][4], b3[4];  // biases of the D rows 16t + 4g + j
 };
 
@@ -165,15 +185,18 @@ __device__ __forceinline__ void load_net(const unsigned short *__restrict__ W, c
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             n.w1[t][j] = short(W[OW1 + (4 * g + j) * NH + 16 * t + r]);
-            n.w3[t][j] = short(W[OW3 + (16 * t + 4 * g + j) * PAD + r]);
             n.b1[t][j] = bias[16 * t + 4 * g + j];
             n.b2[t][j] = bias[NH + 16 * t + 4 * g + j];
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) n.w2[t][c][j] = short(W[OW2 + (16 * c + 4 * g + j) * NH + 16 * t + r]);
+            for (int j = 0; j < 8; ++j) n.w2[t][c][j] = short(W[OW2 + k32_neuron(c, g, j) * NH + 16 * t + r]);
     }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) n.w3[c][j] = short(W[OW3 + k32_neuron(c, g, j) * PAD + r]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) n.b3[j] = bias[2 * NH + 4 * g + j];
 }
@@ -189,17 +212,27 @@ __device__ __forceinline__ floatx4 forward_regs(const NetRegs &n, const bf16x4 &
 #pragma unroll
         for (int j = 0; j < 4; ++j) h1[t][j] = short(to_bf16(gelu(acc[j] + n.b1[t][j])));
     }
+    // K blocks of 32 neurons from the D-layout registers of two output blocks
+    auto kblock = [](const bf16x4 &lo, const bf16x4 &hi) {
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b[j] = lo[j];
+            b[4 + j] = hi[j];
+        }
+        return b;
+    };
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         floatx4 acc = zero;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc = mfma(n.w2[t][c], h1[c], acc);
+        for (int c = 0; c < 2; ++c) acc = mfma32(n.w2[t][c], kblock(h1[2 * c], h1[2 * c + 1]), acc);
 #pragma unroll
         for (int j = 0; j < 4; ++j) h2[t][j] = short(to_bf16(gelu(acc[j] + n.b2[t][j])));
     }
     floatx4 y = zero;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) y = mfma(n.w3[c], h2[c], y);
+    for (int c = 0; c < 2; ++c) y = mfma32(n.w3[c], kblock(h2[2 * c], h2[2 * c + 1]), y);
 #pragma unroll
     for (int j = 0; j < 4; ++j) y[j] += n.b3[j];
     return y;
