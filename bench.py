@@ -390,17 +390,92 @@ def ensemble_bench(B, dt, steps, warmup, autotune):
                 autotuned=autotune, finite=finite, kernel_ms_per_step=kt)
 
 
+def ensemble_identical(B, dt, steps, warmup, copies=4, res=(128, 128, 64)):
+    """`copies` independent 128x128x64 LES (the headline model, seeded noise
+    per member) advanced as one ensemble: one launch per kernel class covers
+    all members.  Grid-point updates per second of the ensemble, to set
+    against the single LES."""
+    W = B.workloads
+    specs = [W.les_neutral(B, res) for _ in range(copies)]
+    ens = B.Ensemble(specs, precision="fp32")
+    for i, m in enumerate(ens.members):
+        n1, n2 = m.default_size()
+        u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2, seed=2000 + i)
+        m.set_physical("vel1", u)
+        m.set_physical("vel2", v)
+        m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
+    ens.init_problem()
+    ens.autotune(reps=3)
+    ens.step(dt, warmup)
+    ms = ens.time_steps(dt, steps)
+    sps = 1e3 * steps / ms
+    finite = all(np.all(np.isfinite(a)) for m in ens.members for a in m.state.values())
+    return dict(instances=copies, grid=list(res), ensemble_steps_per_s=round(sps, 2),
+                ms_per_ensemble_step=round(ms / steps, 4),
+                grid_point_updates_per_s=round(sps * copies * res[0] * res[1] * res[2], 1), finite=finite)
+
+
 MLP_FLOP_PER_POINT = 2 * (10 * 64 + 64 * 64 + 64 * 6)  # forward, useful (unpadded) multiply-adds
 BF16_DENSE_PEAK_TFLOPS = 2500.0
+
+
+TRAINED_CLOSURE = os.path.join(PKG_DIR, "closure_params_dns256.npy")
+
+
+def closure_params(B):
+    """The network trained on the 256x256x128 channel DNS (train_closure.py),
+    or an untrained one if that file is absent: (flat float32 params, label)."""
+    if os.path.exists(TRAINED_CLOSURE):
+        return np.load(TRAINED_CLOSURE).astype(np.float32), "trained on the 256x256x128 fp64 channel DNS"
+    return B.closure.pack(B.closure.init_params(3)), "untrained (seeded initial weights)"
+
+
+PROFILE_NAMES = ("vel1", "vel2", "sgs11", "sgs22", "sgs33", "sgs12", "sgs13", "sgs23")
+
+
+def closure_vs_smagorinsky(B, params, res, dt, steps):
+    """The LES with the learned closure judged against the same LES with
+    Smagorinsky: both from the same seeded state, `steps` SSPRK33 steps with
+    the mean profiles of the reference's MeanProfiles collected on the device;
+    the relative L2 difference of the time-mean profiles (velocity and the
+    SGS stresses) and the mean rough-wall stress of each run."""
+    W = B.workloads
+    means = {}
+    for kind in ("smagorinsky", "learned"):
+        n, dom, procs = W.les_neutral(B, res)
+        if kind == "learned":
+            procs[-1] = B.LearnedSGSModel(params, fallback=procs[-1], guard=True)
+        m = B.Model(n, dom, procs, precision="fp32")
+        n1, n2 = m.default_size()
+        u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2)
+        m.set_physical("vel1", u)
+        m.set_physical("vel2", v)
+        m.set_physical("vel3", np.zeros((n1, n2, m.n3 - 1), order="F"))
+        m.init_problem()
+        m.enable_profiles(PROFILE_NAMES)
+        m.step(dt, steps)
+        integ, k = m.read_profiles()
+        means[kind] = {p: integ[p] / (k * dt) for p in PROFILE_NAMES}
+        means[kind]["finite"] = all(np.all(np.isfinite(a)) for a in m.state.values())
+        del m
+    s, l = means["smagorinsky"], means["learned"]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    return dict(steps=steps, time_span=steps * dt, finite=bool(s["finite"] and l["finite"]),
+                mean_profile_rel_l2={p: round(rel(l[p], s[p]), 5) for p in PROFILE_NAMES},
+                wall_stress_sgs13={"smagorinsky": float(s["sgs13"][0]), "learned": float(l["sgs13"][0])},
+                resolved_vs_sgs13_top_half={"smagorinsky": float(np.mean(s["sgs13"][len(s["sgs13"]) // 2:])),
+                                            "learned": float(np.mean(l["sgs13"][len(l["sgs13"]) // 2:]))})
 
 
 def closure_bench(B, res, dt, steps, warmup):
     """The same LES with the learned SGS closure (bf16 MFMA network at every
     C-node, Smagorinsky fallback and dissipation guard) in fp32, plus the
-    network's standalone forward / training-gradient throughput."""
+    network's standalone forward / training-gradient throughput and the
+    comparison with the Smagorinsky LES."""
     W, CL = B.workloads, B.closure
+    params, label = closure_params(B)
     n, dom, procs = W.les_neutral(B, res)
-    procs[-1] = B.LearnedSGSModel(CL.pack(CL.init_params(3)), fallback=procs[-1], guard=True)
+    procs[-1] = B.LearnedSGSModel(params, fallback=procs[-1], guard=True)
     m = B.Model(n, dom, procs, precision="fp32")
     n1, n2 = m.default_size()
     u, v = W.les_initial(m.coordinates("vel1", 3), n1, n2)
@@ -422,15 +497,19 @@ def closure_bench(B, res, dt, steps, warmup):
     net = CL.MLP(seed=1)
     npts = 1 << 20
     f_ms, g_ms = net.time(npts, reps=10)
-    return dict(steps_per_s=round(1e3 * steps / ms, 2), ms_per_step=round(ms / steps, 4), finite=finite,
-                transform_path=m.tuning()["transform_path"],
+    path = m.tuning()["transform_path"]
+    del m
+    versus = closure_vs_smagorinsky(B, params, res, dt, 400)
+    return dict(network=label, steps_per_s=round(1e3 * steps / ms, 2), ms_per_step=round(ms / steps, 4), finite=finite,
+                transform_path=path,
                 sgs_mlp_avg_us=round(1e3 * ms_mlp / launches, 2), points_per_launch=pts,
                 sgs_mlp_tflops=round(tflops, 2), sgs_mlp_frac_bf16_peak=round(tflops / BF16_DENSE_PEAK_TFLOPS, 4),
                 kernels={k: round(1e3 * v[0] / v[1], 2) for k, v in kt.items() if v[1]},
                 mlp_forward_1M_ms=round(f_ms, 4),
                 mlp_forward_tflops=round(MLP_FLOP_PER_POINT * npts / (f_ms * 1e-3) / 1e12, 2),
                 mlp_gradient_1M_ms=round(g_ms, 4),
-                mlp_gradient_tflops=round(3 * MLP_FLOP_PER_POINT * npts / (g_ms * 1e-3) / 1e12, 2))
+                mlp_gradient_tflops=round(3 * MLP_FLOP_PER_POINT * npts / (g_ms * 1e-3) / 1e12, 2),
+                versus_smagorinsky=versus)
 
 
 def copy_probe(mib=256, reps=10):
@@ -673,6 +752,13 @@ def main():
         log("ensemble ...")
         ens = ensemble_bench(B, args.dt, max(args.steps // 10, 10), 5, bool(args.autotune))
         log("ensemble:", ens)
+    ens4 = None
+    if not args.no_ensemble and res == (128, 128, 64):
+        log("ensemble of 4 headline LES ...")
+        ens4 = ensemble_identical(B, args.dt, max(args.steps // 5, 10), 5)
+        ens4["batching_gain_vs_single"] = round(ens4["grid_point_updates_per_s"] /
+                                                (1e3 / r32["ms_per_step"] * res[0] * res[1] * res[2]), 3)
+        log("ensemble x4:", ens4)
     clo = None
     if not args.no_closure:
         log("learned closure ...")
@@ -713,6 +799,7 @@ def main():
         "fft_autotune": tuning32,
         "pressure_solve_65536x256": pres,
         "ensemble_32_uneven_les": ens,
+        "ensemble_4_headline_les": ens4,
         "learned_closure_les": clo,
         "cpu_baseline": cpu,
     }

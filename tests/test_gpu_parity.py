@@ -437,6 +437,36 @@ def test_ensemble_matches_oracle():
             assert relerr(got[f], ref.state[f]) < RTOL64, (name, f, relerr(got[f], ref.state[f]))
 
 
+def test_ensemble_fp32_members_match_oracle():
+    """Mixed-form ensemble members (uneven LES grids: whole-plane transforms
+    of several lengths in shared launches, float64 (0,0) columns and pressure)
+    against the oracle's float64 steps, within the float32 bound of the
+    headline tests."""
+    names = ["les", "les_square", "les_closed"]
+    refs, specs, states = [], [], []
+    for i, name in enumerate(names):
+        ref = make(O, name)
+        st = random_state(ref, np.random.default_rng(70 + i))
+        for f in ref.fields:
+            ref.state[f][...] = st[f]
+        refs.append(ref)
+        states.append(st)
+        c = CASES[name]
+        dom = B.Domain(c["dims"], _boundary(B, c["lower"]), _boundary(B, c["upper"]))
+        specs.append((c["res"], dom, c["procs"](B)))
+    ens = B.Ensemble(specs, precision="fp32")
+    for m, st in zip(ens.members, states):
+        m.set_state(**st)
+    dt, steps = 5e-4, 3
+    ens.init_problem()
+    ens.step(dt, steps)
+    for ref, m, name in zip(refs, ens.members, names):
+        O.evolve(ref, steps * dt, dt)
+        got = m.state
+        for f in ref.fields:
+            assert relerr(got[f], ref.state[f]) < 1e-5, (name, f, relerr(got[f], ref.state[f]))
+
+
 def test_ensemble_fp32_matches_single_models():
     """Uneven LES grids in one fp32 ensemble equal the same models run alone."""
     res_list = [(12, 12, 6), (16, 16, 8), (20, 20, 10)]
