@@ -10,7 +10,7 @@ from .model import (  # noqa: F401
     MomentumAdvection, MolecularDiffusion, Pressure, ConstantSource, ConstantMean, StaticSmagorinskyModel,
     LearnedSGSModel, SSPRK33, SSPRK22, Euler, AB2,
     incompressible_flow, closedchannelflow, openchannelflow, Model, Ensemble, evolve, pdsize, Snapshots,
-    MeanProfiles,
+    MeanProfiles, StepTimer, ProgressMonitor,
 )
 from . import _lib  # noqa: F401
 from .ldlt import BatchLDLt  # noqa: F401

@@ -809,6 +809,158 @@ class MeanProfiles:
         return fn
 
 
+def _layer_physical(fd, n1, n2):
+    """Physical values of one layer of spectral coefficients (k1 0..K1-1,
+    k2 in the order 0..k2max, -k2max..-1) on an n1 x n2 grid: zero-padded,
+    unnormalised inverse transform (physical_space.jl:277-312)."""
+    K1, K2 = fd.shape
+    k1m, k2m = min(n1 // 2 + 1, K1) - 1, (min(n2, K2) - 1) // 2
+    buf = np.zeros((n1 // 2 + 1, n2), dtype=np.complex128)
+    buf[: k1m + 1, : k2m + 1] = fd[: k1m + 1, : k2m + 1]
+    if k2m > 0:
+        buf[: k1m + 1, n2 - k2m:] = fd[: k1m + 1, K2 - k2m:]
+    return np.fft.irfftn(buf, s=(n2, n1), axes=(1, 0), norm="forward")
+
+
+class StepTimer:
+    """StepTimer output (logging.jl:76-127): the wall time and the simulation
+    time after every `frequency`-th step (and of the initial state), written
+    as JSON to `path` at the end: {"frequency", "wallTime" (seconds since the
+    timer was made), "simulationTime"}.  The engine steps in chunks of
+    `frequency` steps while a StepTimer is active, so each stamp is taken
+    after its step has finished on the GPU."""
+
+    def __init__(self, path=os.path.join("output", "timestamps.json"), frequency=1):
+        import time
+        self.path, self.frequency = path, int(frequency)
+        self.init_time = time.time()
+        self.wall, self.sim = [], []
+
+    def stamp(self, t):
+        import time
+        self.wall.append(time.time() - self.init_time)
+        self.sim.append(float(t))
+
+    def flush(self):
+        d = os.path.dirname(self.path)
+        if d:
+            os.makedirs(d, exist_ok=True)
+        with open(self.path, "w") as io:
+            io.write("{\n")
+            io.write(f'  "frequency": {self.frequency},\n')
+            io.write('  "wallTime": [' + ", ".join(repr(v) for v in self.wall) + "],\n")
+            io.write('  "simulationTime": [' + ", ".join(repr(v) for v in self.sim) + "]\n")
+            io.write("}\n")
+
+
+class ProgressMonitor:
+    """ProgressMonitor output (logging.jl:129-308): at most every `frequency`
+    seconds of wall time (checked after each chunk of steps, and at the end) a
+    report with the progress bar and ETA, the simulation time, the mean
+    velocity (the z-weighted mean of the (0,0) modes), the mean rough-wall
+    stresses where the model has an SGS process, the mean and turbulent
+    kinetic energy (horizontal means of the physical fields, summed over the
+    layers) and the advective Courant number Δt/min(Δx3/max|u3|) (with
+    `tstep`) or the advective time scale; a dot after every chunk in between.
+    Physical values are taken at the default grid (the reference uses the
+    largest physical grid of the model).  `out` receives the text."""
+
+    def __init__(self, frequency=10, description="Progress:", tstep=None, out=None, chunk=64):
+        import sys
+        self.frequency, self.description, self.tstep = frequency, description, tstep
+        self.out = sys.stdout if out is None else out
+        self.chunk = int(chunk)
+        self.timestamp = math.nan
+        self.progress = math.nan
+        self.remaining = math.nan
+        self.reports = []
+
+    def start(self, t1, t2):
+        self.range = (t1, t2)
+
+    def samples(self, model, t):
+        d, n3 = model.domain, model.n3
+        dz = lambda z: d.Dvmap(z) / n3
+        cw = np.array([dz(z) for z in _zeta(n3, "C")]) / d.size(3)
+        iw = np.array([dz(z) for z in _zeta(n3, "I")]) / d.size(3)
+        st = model.state
+        out = [("Simulation Time", (t,))]
+        means = []
+        for f, w in (("vel1", cw), ("vel2", cw), ("vel3", iw)):
+            means.append(float(np.sum(st[f][0, 0, :].real * w)) if f in st else 0.0)
+        out.append(("Mean Velocity", tuple(means)))
+        sgs = [p for p in model.processes if isinstance(p, (StaticSmagorinskyModel, LearnedSGSModel))]
+        if sgs:
+            # rough-wall stress at the first C-layer next to each wall, on the
+            # dealiased grid of the SGS terms (static_smagorinsky.jl:176-184)
+            c = model.disc.config
+            vals = {0: [], 1: []}
+            for side, lev in ((0, 0), (1, n3 - 1)):
+                if c.sgs_wall[side] == _lib.BLD_WALL_ROUGH:
+                    a = _layer_physical(st["vel1"][:, :, lev], c.sgs_n1, c.sgs_n2)
+                    b = _layer_physical(st["vel2"][:, :, lev], c.sgs_n1, c.sgs_n2)
+                    mag = np.sqrt(a * a + b * b)
+                    vals[0].append(float(np.mean(c.sgs_factor_stress[side] * a * mag)))
+                    vals[1].append(float(np.mean(c.sgs_factor_stress[side] * b * mag)))
+                else:
+                    vals[0].append(0.0)
+                    vals[1].append(0.0)
+            out.append(("Wall Stress τ₁₃", tuple(vals[0])))
+            out.append(("Wall Stress τ₂₃", tuple(vals[1])))
+        u, v = model.get_physical("vel1"), model.get_physical("vel2")
+        u1 = u.mean(axis=(0, 1))
+        u2 = v.mean(axis=(0, 1))
+        mke = u1 ** 2 + u2 ** 2
+        ke = (u ** 2).mean(axis=(0, 1)) + (v ** 2).mean(axis=(0, 1))
+        out.append(("Mean KE", (float(np.sum(mke)),)))
+        out.append(("Turbulent KE", (float(np.sum(ke - mke)),)))
+        w = model.get_physical("vel3")
+        dx3i = np.array([dz(z) for z in _zeta(n3, "I")])
+        wmax = np.abs(w).max(axis=(0, 1))
+        with np.errstate(divide="ignore"):
+            dtmin = float(np.min(dx3i / wmax)) if wmax.size else math.inf
+        if self.tstep is not None:
+            out.append(("Adv. Courant Number", (self.tstep / dtmin,)))
+        else:
+            out.append(("Advective Timescale", (dtmin,)))
+        return out
+
+    def check(self, model, t):
+        """prepare_samples!, log_state! and process_log! (logging.jl:188-306)
+        at time t after a chunk of steps: a report when due, else a dot."""
+        import time
+        progress = (t - self.range[0]) / (self.range[1] - self.range[0]) if self.range[1] > self.range[0] else 1.0
+        now = time.time()
+        due = math.isnan(self.timestamp) or now - self.timestamp >= self.frequency or progress >= 1
+        if not due:
+            self.out.write("·")
+            self.out.flush()
+            return None
+        self.out.write("\n")
+        rate = (progress - self.progress) / (now - self.timestamp) if not math.isnan(self.timestamp) else math.nan
+        self.remaining = (1 - progress) / rate if rate and rate > 0 else math.nan
+        self.timestamp, self.progress = now, progress
+        samples = self.samples(model, t)
+        eta = "unknown"
+        if self.remaining < 9e9:
+            sec = int(math.ceil(self.remaining))
+            eta = f"{sec // 3600}:{(sec // 60) % 60:02d}:{sec % 60:02d}"
+        L, blocks = 25, " ▏▎▍▌▋▊▉█"
+        ip = int(round(progress * L * 8))
+        bar = "".join(blocks[min(max(1 + ip - i, 1), 9) - 1] for i in range(0, 8 * L, 8))
+        msg = f"{self.description} {100 * progress:3.0f}%▕{bar}▏ ETA: {eta}"
+        wt = len(msg)
+        wl = max(len(k) for k, _ in samples) + 2
+        lines = ["─" * wt, msg, "─" * wl + "┬" + "─" * (wt - wl - 1)]
+        for label, vals in samples:
+            lines.append(label.rjust(wl - 1) + " ╎ " + ", ".join(f"{v:.3g}" for v in vals))
+        lines.append("─" * wl + "┴" + "─" * (wt - wl - 1))
+        self.out.write("\n".join(lines) + "\n")
+        self.out.flush()
+        self.reports.append(dict(samples))
+        return samples
+
+
 class Snapshots:
     """Snapshots output (logging.jl:526-602): the state as CBD files in
     `path`/state-NNN/ at every multiple NNN of `frequency` (the initial state is
@@ -841,7 +993,7 @@ def evolve(model, tspan, dt, method=None, graph=True, output=()):
         raise ValueError("Integration start time is larger than end time")
     outputs = list(output) if isinstance(output, (list, tuple)) else [output]
     for o in outputs:
-        if not isinstance(o, (Snapshots, MeanProfiles)):
+        if not isinstance(o, (Snapshots, MeanProfiles, StepTimer, ProgressMonitor)):
             raise NotImplementedError(f"output {type(o).__name__} is not supported by the GPU engine")
     if sum(isinstance(o, MeanProfiles) for o in outputs) > 1:
         raise NotImplementedError("one MeanProfiles output per model")
@@ -849,30 +1001,47 @@ def evolve(model, tspan, dt, method=None, graph=True, output=()):
     for o in outputs:
         if isinstance(o, MeanProfiles):
             o.start(model, t1)
+        elif isinstance(o, StepTimer):
+            o.stamp(t1)
+        elif isinstance(o, ProgressMonitor):
+            o.start(t1, t2)
+            o.check(model, t1)
     if t2 <= t1:
         return t1
     nt = _approxdiv(t2 - t1, dt)
     if not outputs:
         return model.step(dt, nt, t=t1, t_end=t2, graph=graph)
-    # step to each output time in one engine call
+    timers = [o for o in outputs if isinstance(o, StepTimer)]
+    monitors = [o for o in outputs if isinstance(o, ProgressMonitor)]
+    timed = [o for o in outputs if isinstance(o, (Snapshots, MeanProfiles))]
+    # step to each output time (and each StepTimer stamp or ProgressMonitor
+    # check) in one engine call
     t, done = t1, 0
     while done < nt:
         k = 1
-        while done + k < nt and not any(o.due(t1 + (done + k) * dt) for o in outputs):
+        limit = min([nt - done] + [o.frequency - done % o.frequency for o in timers] + [m.chunk for m in monitors])
+        while k < limit and not any(o.due(t1 + (done + k) * dt) for o in timed):
             k += 1
         t = model.step(dt, k, t=t, t_end=t2, graph=graph)
         done += k
-        for o in outputs:
+        for o in timed:
             if isinstance(o, Snapshots):
                 n = o.due(t)
                 if n is not None:
                     o.write(model, n)
             elif o.due(t):
                 o.flush(model, t)
+        for o in timers:
+            if done % o.frequency == 0:
+                o.stamp(t)
+        for o in monitors:
+            o.check(model, t)
     for o in outputs:  # flush! at the end of the run (logging.jl:453-491)
         if isinstance(o, MeanProfiles):
             o.flush(model, t)
             model.enable_profiles(())
+        elif isinstance(o, StepTimer):
+            o.flush()
     return t
 
 
