@@ -467,11 +467,11 @@ def closure_vs_smagorinsky(B, params, res, dt, steps):
                                             "learned": float(np.mean(l["sgs13"][len(l["sgs13"]) // 2:]))})
 
 
-def closure_bench(B, res, dt, steps, warmup):
+def closure_bench(B, res, dt, steps, warmup, versus=True):
     """The same LES with the learned SGS closure (bf16 MFMA network at every
     C-node, Smagorinsky fallback and dissipation guard) in fp32, plus the
-    network's standalone forward / training-gradient throughput and the
-    comparison with the Smagorinsky LES."""
+    network's standalone forward / training-gradient throughput and (with
+    `versus`) the comparison with the Smagorinsky LES."""
     W, CL = B.workloads, B.closure
     params, label = closure_params(B)
     n, dom, procs = W.les_neutral(B, res)
@@ -499,7 +499,7 @@ def closure_bench(B, res, dt, steps, warmup):
     f_ms, g_ms = net.time(npts, reps=10)
     path = m.tuning()["transform_path"]
     del m
-    versus = closure_vs_smagorinsky(B, params, res, dt, 400)
+    versus = closure_vs_smagorinsky(B, params, res, dt, 400) if versus else None
     return dict(network=label, steps_per_s=round(1e3 * steps / ms, 2), ms_per_step=round(ms / steps, 4), finite=finite,
                 transform_path=path,
                 sgs_mlp_avg_us=round(1e3 * ms_mlp / launches, 2), points_per_launch=pts,
@@ -678,6 +678,20 @@ def measured_traffic(kernel):
     return int(tot / n) if n else None
 
 
+def counter_summary(directory):
+    """Markdown table of every counter of a rocprofv3 --pmc run: the mean per
+    dispatch of each kernel over the later half of its dispatches."""
+    names = sorted({r["Counter_Name"] for r in _csv_rows(directory, "counter_collection.csv")})
+    per = {n: _counter_per_kernel(directory, n) for n in names}
+    kernels = sorted({k for v in per.values() for k in v})
+    lines = ["| kernel | dispatches | " + " | ".join(names) + " |", "|---|---|" + "---|" * len(names)]
+    for k in kernels:
+        cnt = max(per[n][k][1] for n in names if k in per[n])
+        vals = [f"{per[n][k][0]:.6g}" if k in per[n] else "" for n in names]
+        lines.append(f"| {k} | {cnt} | " + " | ".join(vals) + " |")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -697,11 +711,17 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the counter-scale copy kernel")
     ap.add_argument("--pressure-only", action="store_true",
                     help="only the isolated pressure solve (65,536 systems x 256 levels), for profiling")
+    ap.add_argument("--closure-only", action="store_true",
+                    help="only the learned-closure LES leg without the Smagorinsky comparison, for profiling")
+    ap.add_argument("--counter-summary", metavar="DIR", help="summarise the counters of a rocprofv3 --pmc run")
     ap.add_argument("--rocprof-summary", nargs=5, metavar="DIR",
                     help="summarise rocprofv3 output: kernel-trace, FETCH_SIZE, WRITE_SIZE, copy FETCH, copy WRITE")
     args = ap.parse_args()
     if args.rocprof_summary:
         rocprof_summary(*args.rocprof_summary, copy_bytes=256 * 2 ** 20)
+        return
+    if args.counter_summary:
+        counter_summary(args.counter_summary)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path "
@@ -715,6 +735,10 @@ def main():
         return
     res = tuple(int(x) for x in args.res.split(","))
     B = load_package()
+    if args.closure_only:
+        print(json.dumps({"learned_closure_les": closure_bench(B, res, args.dt, max(args.steps // 3, 10), 5,
+                                                               versus=False)}), flush=True)
+        return
     log("fp32 mixed form ...")
     m32, r32, kt32 = time_form(B, "fp32", res, args.dt, args.steps, args.warmup, args.profile_steps,
                                autotune=bool(args.autotune))

@@ -21,23 +21,31 @@ constexpr int OW1 = 0, OW2 = OW1 + PAD * NH, OW3 = OW2 + NH * NH, OB1 = OW3 + NH
 // bfloat16 weight copy: W1, W2, W3 in the same row-major layout
 constexpr int NWB = OB1;
 
-typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float floatx4 __attribute__((__vector_size__(4 * sizeof(float))));
 
-// round to nearest even, as bits
-__device__ __forceinline__ unsigned short to_bf16(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// round to nearest even, as bits (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ unsigned short to_bf16(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float floatv2 __attribute__((ext_vector_type(2)));
+typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
+// four values rounded to nearest even: two v_cvt_pk_bf16_f32
+__device__ __forceinline__ bf16x4 to_bf16x4(float a, float b, float c, float d) {
+    const bf16v2 lo = __builtin_convertvector((floatv2){a, b}, bf16v2);
+    const bf16v2 hi = __builtin_convertvector((floatv2){c, d}, bf16v2);
+    typedef unsigned u32x2 __attribute__((__vector_size__(2 * sizeof(unsigned))));
+    return __builtin_bit_cast(bf16x4, (u32x2){__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi)});
 }
 __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float(uint32_t(b) << 16); }
 
 // GELU, tanh form: z/2·(1 + tanh(u)), u = √(2/π)(z + 0.044715 z³).  With
 // 1 + tanh(u) = 2σ(2u), σ the logistic function, one exp and one reciprocal
 // give it (exp overflow for very negative z gives σ = 0, as it should).
+// The exponential is taken as a power of two, 2u·log2(e) = z·(k + k·0.044715·z²)
+// with k = 2·sqrt(2/π)·log2(e).
 __device__ __forceinline__ float gelu_sigma(float z) {
-    const float s2 = 2.0f * 0.7978845608028654f;  // 2·sqrt(2/π)
-    const float u2 = s2 * (z + 0.044715f * z * z * z);
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-u2));
+    constexpr float k = 2.0f * 0.7978845608028654f * 1.4426950408889634f;
+    const float m = __builtin_fmaf(-k * 0.044715f, z * z, -k);  // -(2u log2 e) / z
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * m));
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z); }
 // d/dz: σ + z·σ(1−σ)·2√(2/π)(1 + 3·0.044715 z²)
@@ -210,7 +218,8 @@ __device__ __forceinline__ floatx4 forward_regs(const NetRegs &n, const bf16x4 &
     for (int t = 0; t < 4; ++t) {
         floatx4 acc = mfma(n.w1[t], xb, zero);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h1[t][j] = short(to_bf16(gelu(acc[j] + n.b1[t][j])));
+        for (int j = 0; j < 4; ++j) acc[j] = gelu(acc[j] + n.b1[t][j]);
+        h1[t] = to_bf16x4(acc[0], acc[1], acc[2], acc[3]);
     }
     // K blocks of 32 neurons from the D-layout registers of two output blocks
     auto kblock = [](const bf16x4 &lo, const bf16x4 &hi) {
@@ -228,7 +237,8 @@ __device__ __forceinline__ floatx4 forward_regs(const NetRegs &n, const bf16x4 &
 #pragma unroll
         for (int c = 0; c < 2; ++c) acc = mfma32(n.w2[t][c], kblock(h1[2 * c], h1[2 * c + 1]), acc);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) h2[t][j] = short(to_bf16(gelu(acc[j] + n.b2[t][j])));
+        for (int j = 0; j < 4; ++j) acc[j] = gelu(acc[j] + n.b2[t][j]);
+        h2[t] = to_bf16x4(acc[0], acc[1], acc[2], acc[3]);
     }
     floatx4 y = zero;
 #pragma unroll
