@@ -877,7 +877,8 @@ class Engine : public EngineBase {
             LaunchEntry E{};
             E.space = int(i);
             E.block_begin = lin_pass.nblocks;
-            E.blocks_per_job = int((size_t(I.nk) * size_t(I.n3) + 255) / 256);  // one job per field
+            // one job per field, a thread per kLinLevels levels of a mode
+            E.blocks_per_job = int((size_t(I.nk) * size_t((I.n3 + kLinLevels - 1) / kLinLevels) + 255) / 256);
             E.njobs = 3;
             lin_pass.ent.push_back(E);
             lin_pass.nblocks += E.blocks_per_job * E.njobs;

@@ -603,70 +603,81 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
     const LinDesc<T> &L = descs[E.space];
     const int nk = L.nk, n3 = L.n3;
     // one job per field: the field index is uniform over the block, so the
-    // descriptor fields it selects are scalar loads
+    // descriptor fields it selects are scalar loads; a thread takes kLinLevels
+    // consecutive levels of one mode (modes fastest over the threads)
     const int bj = blockIdx.x - E.block_begin;
     const int f = bj / E.blocks_per_job;
     const int nl = f < 2 ? n3 : n3 - 1;
+    const uint32_t nch = uint32_t(nl + kLinLevels - 1) / kLinLevels;
     const uint32_t r = uint32_t(bj - f * E.blocks_per_job) * blockDim.x + threadIdx.x;
-    if (r >= uint32_t(nl) * uint32_t(nk)) return;
-    const int l = int(r / uint32_t(nk)), m = int(r - uint32_t(l) * nk);
+    if (r >= nch * uint32_t(nk)) return;
+    const int c = int(r / uint32_t(nk)), m = int(r - uint32_t(c) * nk);
+    const int l0 = c * kLinLevels;
     const cplx<T> *q = L.state[sel][f];
     cplx<T> *du = L.du[f];
-    const bool mixed00 = (m == 0) && (L.shadow[sel][f] != nullptr);
-    if (mixed00) {
+    if (m == 0 && L.shadow[sel][f] != nullptr) {
         // (0,0) mode of the mixed form: float64 from the float64 shadow
         const cplx<double> *qs = L.shadow[sel][f];
-        cplx<double> res = mk(0.0, 0.0);
-        if (L.has_diff[f]) {
-            cplx<double> lo, hi, c = qs[l];
-            if (f < 2) {
-                lo = l > 0 ? qs[l - 1] : ghost<double>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], true, false, qs[0], qs[1], qs[2]);
-                hi = l < nl - 1 ? qs[l + 1]
-                                : ghost<double>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], true, true, qs[nl - 1], qs[nl - 2], qs[nl - 3]);
-            } else {
-                lo = l > 0 ? qs[l - 1] : mk(L.bc_val[2][0], 0.0);
-                hi = l < nl - 1 ? qs[l + 1] : mk(L.bc_val[2][1], 0.0);
+        for (int l = l0; l < min(l0 + kLinLevels, nl); ++l) {
+            cplx<double> res = mk(0.0, 0.0);
+            if (L.has_diff[f]) {
+                cplx<double> lo, hi, cc = qs[l];
+                if (f < 2) {
+                    lo = l > 0 ? qs[l - 1] : ghost<double>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], true, false, qs[0], qs[1], qs[2]);
+                    hi = l < nl - 1 ? qs[l + 1]
+                                    : ghost<double>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], true, true, qs[nl - 1], qs[nl - 2], qs[nl - 3]);
+                } else {
+                    lo = l > 0 ? qs[l - 1] : mk(L.bc_val[2][0], 0.0);
+                    hi = l < nl - 1 ? qs[l + 1] : mk(L.bc_val[2][1], 0.0);
+                }
+                res = laplacian_point<T, double>(L, f, l, m, cc, lo, hi);
             }
-            res = laplacian_point<T, double>(L, f, l, m, c, lo, hi);
+            res.x += L.src[f];
+            L.du00[f][l] = res;
+            du[size_t(l) * nk] = mk(T(0), T(0));
         }
-        res.x += L.src[f];
-        L.du00[f][l] = res;
-        du[size_t(l) * nk] = mk(T(0), T(0));
         return;
     }
     const bool diff = L.has_diff[f] != 0;
     const T src = T(L.src[f]);
-    const size_t i = size_t(l) * nk + m;
-    cplx<T> res = mk(T(0), T(0));
+    // the column window: levels l0-1 .. l0+kLinLevels at clamped indices, and
+    // the second level from a boundary for the ghost layers
+    cplx<T> w[kLinLevels + 2], qx0 = mk(T(0), T(0)), qx1 = mk(T(0), T(0));
+    T dd = T(0);
     if (diff) {
-        // the column values a thread needs, loaded together with clamped
-        // levels: below, at and above l, and the second level from the
-        // boundary for the ghost layers
-        const int llo = max(l - 1, 0), lhi = min(l + 1, nl - 1), lx = l == 0 ? min(2, nl - 1) : max(l - 2, 0);
-        const cplx<T> c = gload(q + i);
-        const cplx<T> qlo = gload(q + size_t(llo) * nk + m);
-        const cplx<T> qhi = gload(q + size_t(lhi) * nk + m);
-        const cplx<T> qx = gload(q + size_t(lx) * nk + m);
+#pragma unroll
+        for (int i = 0; i < kLinLevels + 2; ++i) w[i] = gload(q + size_t(min(max(l0 - 1 + i, 0), nl - 1)) * nk + m);
+        qx0 = gload(q + size_t(min(2, nl - 1)) * nk + m);
+        qx1 = gload(q + size_t(max(nl - 3, 0)) * nk + m);
         const int k1 = m % L.K1, i2 = m / L.K1;
-        const T dd = T(gload(L.DD1 + k1)) + T(gload(L.DD2 + i2));
-        const double *pa = f < 2 ? L.alpha_i : L.alpha_c, *pb = f < 2 ? L.alpha_c : L.alpha_i;
-        const T a0 = T(gload(pa + l)), a1 = T(gload(pb + l + (f < 2 ? 0 : 1))), a2 = T(gload(pa + l + 1));
-        const T nu = T(L.nu[f]);
-        cplx<T> lo, hi;
-        if (f < 2) {
-            lo = l > 0 ? qlo : ghost<T>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], m == 0, false, c, qhi, qx);
-            hi = l < nl - 1 ? qhi : ghost<T>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], m == 0, true, c, qlo, qx);
-        } else {
-            // I-nodes: Dirichlet values enter every mode (boundary_conditions.jl:263-274)
-            lo = l > 0 ? qlo : mk(T(L.bc_val[2][0]), T(0));
-            hi = l < nl - 1 ? qhi : mk(T(L.bc_val[2][1]), T(0));
-        }
-        const T c0 = a0 * a1, c2 = a1 * a2;
-        const T c1 = dd - a0 * a1 - a1 * a2;
-        res = mk(nu * (c0 * lo.x + c1 * c.x + c2 * hi.x), nu * (c0 * lo.y + c1 * c.y + c2 * hi.y));
+        dd = T(gload(L.DD1 + k1)) + T(gload(L.DD2 + i2));
     }
-    if (m == 0) res.x += src;
-    gstore(du + i, res);
+    const double *pa = f < 2 ? L.alpha_i : L.alpha_c, *pb = f < 2 ? L.alpha_c : L.alpha_i;
+    const T nu = T(L.nu[f]);
+#pragma unroll
+    for (int j = 0; j < kLinLevels; ++j) {
+        const int l = l0 + j;
+        if (l >= nl) break;
+        cplx<T> res = mk(T(0), T(0));
+        if (diff) {
+            const cplx<T> cc = w[j + 1], qlo = w[j], qhi = w[j + 2];
+            const T a0 = T(gload(pa + l)), a1 = T(gload(pb + l + (f < 2 ? 0 : 1))), a2 = T(gload(pa + l + 1));
+            cplx<T> lo, hi;
+            if (f < 2) {
+                lo = l > 0 ? qlo : ghost<T>(L.bc_kind[f][0], L.bc_val[f][0], L.bc_diff[f][0], m == 0, false, cc, qhi, qx0);
+                hi = l < nl - 1 ? qhi : ghost<T>(L.bc_kind[f][1], L.bc_val[f][1], L.bc_diff[f][1], m == 0, true, cc, qlo, qx1);
+            } else {
+                // I-nodes: Dirichlet values enter every mode (boundary_conditions.jl:263-274)
+                lo = l > 0 ? qlo : mk(T(L.bc_val[2][0]), T(0));
+                hi = l < nl - 1 ? qhi : mk(T(L.bc_val[2][1]), T(0));
+            }
+            const T c0 = a0 * a1, c2 = a1 * a2;
+            const T c1 = dd - a0 * a1 - a1 * a2;
+            res = mk(nu * (c0 * lo.x + c1 * cc.x + c2 * hi.x), nu * (c0 * lo.y + c1 * cc.y + c2 * hi.y));
+        }
+        if (m == 0) res.x += src;
+        gstore(du + size_t(l) * nk + m, res);
+    }
 }
 
 // Divergence record (the isfinite check of BoundaryLayerDynamics.jl:155-157):

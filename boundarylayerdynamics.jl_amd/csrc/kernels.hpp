@@ -18,6 +18,8 @@ constexpr int kC2r2dThreads = 1024;  // block of the fused r2c (one plane per bl
 enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2, PHYS_SGS_MLP = 3, PHYS_SGS_MLP_STRESS = 4 };
 // C-node points per block of the learned-closure kernel (256 threads x chunks)
 constexpr int kMlpChunks = 4;
+// consecutive levels of one mode per thread of the linear-term kernel
+constexpr int kLinLevels = 4;
 
 // Spectral linear terms of one model instance.
 template <typename T>
