@@ -855,24 +855,25 @@ class Engine : public EngineBase {
         reset(lin_pass);
         for (size_t i = 0; i < inst.size(); ++i) {
             Instance<T> &I = *inst[i];
-            auto add_points = [&](Pass &ps, int sidx, int per_block = 256) {
+            // per_block points per block; a thread per `levels` C-levels of a column
+            auto add_points = [&](Pass &ps, int sidx, int per_block = 256, int levels = 1) {
                 const Space<T> &S = spaces[sidx];
                 LaunchEntry E{};
                 E.space = sidx;
                 E.block_begin = ps.nblocks;
-                const size_t pts = size_t(S.N1) * S.N2 * S.n3;
+                const size_t pts = size_t(S.N1) * S.N2 * size_t((S.n3 + levels - 1) / levels);
                 E.blocks_per_job = int((pts + per_block - 1) / per_block);
                 E.njobs = 1;
                 ps.ent.push_back(E);
                 ps.nblocks += E.blocks_per_job;
             };
-            if (I.adv >= 0) add_points(pp[PHYS_ADV], I.adv);
+            if (I.adv >= 0) add_points(pp[PHYS_ADV], I.adv, 256, kColLevels);
             if (I.sgs >= 0 && I.closure) {
                 add_points(pp[PHYS_SGS_MLP], I.sgs, 256 * kMlpChunks);
                 add_points(pp[PHYS_SGS_MLP_STRESS], I.sgs);
             } else if (I.sgs >= 0) {
-                add_points(pp[PHYS_SGS_VISC], I.sgs);
-                add_points(pp[PHYS_SGS_STRESS], I.sgs);
+                // eddy viscosity and stresses in one kernel (PHYS_SGS_VISC stays empty)
+                add_points(pp[PHYS_SGS_STRESS], I.sgs, 256, kSgsLevels);
             }
             LaunchEntry E{};
             E.space = int(i);
@@ -911,7 +912,10 @@ class Engine : public EngineBase {
             stage_pass.nblocks += E.blocks_per_job;
             projs.push_back(I->proj);
             if (I->mixed) {
-                I->proj00.MB = 256 / NC;
+                const int cz = stage00_cl(stage_CL);
+                int NCz = 1;
+                while (NCz < (I->n3 + cz - 1) / cz) NCz *= 2;
+                I->proj00.MB = 256 / NCz;
                 LaunchEntry Z{};
                 Z.space = int(projs00.size());
                 Z.block_begin = stage00_pass.nblocks;

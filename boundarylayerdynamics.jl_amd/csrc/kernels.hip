@@ -364,6 +364,9 @@ __global__ void __launch_bounds__(256) k_r2c_x2(const Space<T> *__restrict__ spa
 // ---------------------------------------------------------------------------
 // advection products (momentum_advection.jl:179-206); w = 0 at the walls
 // ---------------------------------------------------------------------------
+// A thread walks kColLevels consecutive C-levels of one column x (columns
+// fastest over the threads): the I-level values above a level and the C-level
+// values of the next level are carried to the next level in registers.
 template <typename T>
 __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
@@ -371,32 +374,44 @@ __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict
     const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
     const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
     const int n3 = S.n3;
-    if (gid >= P * uint32_t(n3)) return;
-    const int l = int(gid / P);
-    const uint32_t x = gid - uint32_t(l) * P;
+    const uint32_t nch = uint32_t(n3 + kColLevels - 1) / kColLevels;
+    if (gid >= P * nch) return;
+    const int c = int(gid / P);
+    const uint32_t x = gid - uint32_t(c) * P;
+    const int l0 = c * kColLevels, l1 = min(l0 + kColLevels, n3);
     const T *u = S.in_phys[0], *v = S.in_phys[1], *w = S.in_phys[2];
     const T *o1 = S.in_phys[3], *o2 = S.in_phys[4], *o3 = S.in_phys[5];
     T *a1 = S.src_phys[0], *a2 = S.src_phys[1], *a3 = S.src_phys[2];
-    // all loads first, at clamped levels: I-levels l-1 and l, C-level l+1
-    const uint32_t i = gid;
-    const uint32_t jb = uint32_t(max(l - 1, 0)) * P + x, ja = uint32_t(min(l, n3 - 2)) * P + x;
-    const uint32_t in = uint32_t(min(l + 1, n3 - 1)) * P + x;
-    const T ul = gload(u + i), vl = gload(v + i), o3l = gload(o3 + i);
-    const T o2b = gload(o2 + jb), o1b = gload(o1 + jb), wb = gload(w + jb);
-    const T o2a = gload(o2 + ja), o1a = gload(o1 + ja), wa = gload(w + ja);
-    const T un = gload(u + in), vn = gload(v + in);
-    T p2b = T(0), p2a = T(0), p1b = T(0), p1a = T(0);
-    if (l > 0) {
-        p2b = o2b * wb;
-        p1b = o1b * wb;
+    // products on the I-level below the first level (zero at the wall) and the
+    // C-values of the first level
+    T p2b = T(0), p1b = T(0);
+    if (l0 > 0) {
+        const uint32_t jb = uint32_t(l0 - 1) * P + x;
+        const T wb = gload(w + jb);
+        p2b = gload(o2 + jb) * wb;
+        p1b = gload(o1 + jb) * wb;
     }
-    if (l < n3 - 1) {
-        p2a = o2a * wa;
-        p1a = o1a * wa;
-        gstore(a3 + ja, o2a * T(0.5) * (ul + un) - o1a * T(0.5) * (vl + vn));
+    T ul = gload(u + uint32_t(l0) * P + x), vl = gload(v + uint32_t(l0) * P + x);
+    for (int l = l0; l < l1; ++l) {
+        const uint32_t i = uint32_t(l) * P + x;
+        const uint32_t ja = uint32_t(min(l, n3 - 2)) * P + x;
+        const uint32_t in = uint32_t(min(l + 1, n3 - 1)) * P + x;
+        const T o3l = gload(o3 + i);
+        const T o2a = gload(o2 + ja), o1a = gload(o1 + ja), wa = gload(w + ja);
+        const T un = gload(u + in), vn = gload(v + in);
+        T p2a = T(0), p1a = T(0);
+        if (l < n3 - 1) {
+            p2a = o2a * wa;
+            p1a = o1a * wa;
+            gstore(a3 + ja, o2a * T(0.5) * (ul + un) - o1a * T(0.5) * (vl + vn));
+        }
+        gstore(a1 + i, o3l * vl - T(0.5) * (p2b + p2a));
+        gstore(a2 + i, T(0.5) * (p1b + p1a) - o3l * ul);
+        p2b = p2a;
+        p1b = p1a;
+        ul = un;
+        vl = vn;
     }
-    gstore(a1 + i, o3l * vl - T(0.5) * (p2b + p2a));
-    gstore(a2 + i, T(0.5) * (p1b + p1a) - o3l * ul);
 }
 
 // ---------------------------------------------------------------------------
@@ -410,18 +425,19 @@ __global__ void __launch_bounds__(256) k_adv_products(const Space<T> *__restrict
 // components are then formed from registers.
 template <typename T>
 struct SgsPoint {
-    const Space<T> &S;
+    const Space<T> *S;
     int l, n3;
     T u[3], v[3];           // C-levels l-1, l, l+1
     T w[3];                 // I-levels l-1, l, l+1
     T u1[2], v2[2], u2[2], v1[2];  // C-levels l, l+1
     T w1[2], w2[2];         // I-levels l-1, l
     T ac[2], ai[2];         // alpha_c[l], alpha_c[l+1]; alpha_i[l], alpha_i[l+1]
-    __device__ SgsPoint(const Space<T> &s, uint32_t P, uint32_t x, int lev) : S(s), l(lev), n3(s.n3) {
+    SgsPoint() = default;
+    __device__ SgsPoint(const Space<T> &s, uint32_t P, uint32_t x, int lev) : S(&s), l(lev), n3(s.n3) {
         const uint32_t cm = uint32_t(max(l - 1, 0)) * P + x, c0 = uint32_t(l) * P + x;
         const uint32_t cp = uint32_t(min(l + 1, n3 - 1)) * P + x;
         const uint32_t i0 = uint32_t(min(l, n3 - 2)) * P + x, ip = uint32_t(min(l + 1, n3 - 2)) * P + x;
-        const T *const *f = S.in_phys;
+        const T *const *f = s.in_phys;
         u[0] = gload(f[0] + cm), u[1] = gload(f[0] + c0), u[2] = gload(f[0] + cp);
         v[0] = gload(f[1] + cm), v[1] = gload(f[1] + c0), v[2] = gload(f[1] + cp);
         w[0] = gload(f[2] + cm), w[1] = gload(f[2] + i0), w[2] = gload(f[2] + ip);
@@ -431,14 +447,14 @@ struct SgsPoint {
         v1[0] = gload(f[6] + c0), v1[1] = gload(f[6] + cp);
         w1[0] = gload(f[7] + cm), w1[1] = gload(f[7] + i0);
         w2[0] = gload(f[8] + cm), w2[1] = gload(f[8] + i0);
-        ac[0] = T(gload(S.alpha_c + l)), ac[1] = T(gload(S.alpha_c + min(l + 1, n3 - 1)));
-        ai[0] = T(gload(S.alpha_i + l)), ai[1] = T(gload(S.alpha_i + l + 1));
+        ac[0] = T(gload(s.alpha_c + l)), ac[1] = T(gload(s.alpha_c + min(l + 1, n3 - 1)));
+        ai[0] = T(gload(s.alpha_i + l)), ai[1] = T(gload(s.alpha_i + l + 1));
     }
     // S33 on C-layer l (k = 0) or l+1 (k = 1) (vel3_3 via dx3_i2c!, physical Dirichlet values)
     __device__ T s33(int k) const {
         const int lk = l + k;
-        const T wb = lk > 0 ? w[k] : T(S.wbc[0]);
-        const T wa = lk < n3 - 1 ? w[k + 1] : T(S.wbc[1]);
+        const T wb = lk > 0 ? w[k] : T(S->wbc[0]);
+        const T wa = lk < n3 - 1 ? w[k + 1] : T(S->wbc[1]);
         return (-ac[k]) * wb + ac[k] * wa;
     }
     __device__ T s12(int k) const { return (u2[k] + v1[k]) / T(2); }
@@ -447,18 +463,12 @@ struct SgsPoint {
     __device__ T s23(int k) const { return ((-ai[k]) * v[k] + ai[k] * v[k + 1] + w2[k]) / T(2); }
 };
 
+// Eddy viscosities of point q: at its C-node (evc) and at the I-node above it
+// (evi; 0 on the top C-layer, which has none).
 template <typename T>
-__global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restrict__ spaces, LaunchTable lt) {
-    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
-    const Space<T> &S = spaces[E.space];
-    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
-    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    if (gid >= P * uint32_t(S.n3)) return;
-    const int l = int(gid / P);
-    const uint32_t x = gid - uint32_t(l) * P;
-    const int n3 = S.n3;
+__device__ __forceinline__ void sgs_visc_point(const Space<T> &S, const SgsPoint<T> &q, T &evc, T &evi) {
+    const int l = q.l, n3 = S.n3;
     const T lcl = T(gload(S.lc + l)), lil = T(gload(S.li + min(l, n3 - 2)));
-    const SgsPoint<T> q(S, P, x, l);
     // C-nodes: total_strain! (:208, :225-229)
     const T S11 = q.u1[0], S22 = q.v2[0], S33 = q.s33(0), S12 = q.s12(0);
     T sc = S11 * S11 + S22 * S22 + S33 * S33 + T(2) * S12 * S12;
@@ -480,7 +490,8 @@ __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restric
         }
     }
     sc = sqrt(T(2) * sc);
-    gstore(S.visc_c + gid, sc * (lcl * lcl));
+    evc = sc * (lcl * lcl);
+    evi = T(0);
     // I-nodes (:209-220)
     if (l < n3 - 1) {
         const T a13 = q.s13(1), a23 = q.s23(1);
@@ -495,37 +506,29 @@ __global__ void __launch_bounds__(256) k_sgs_viscosity(const Space<T> *__restric
         s = S12 + q.s12(1);
         si += T(2) * (s * s) / T(4);
         si = sqrt(T(2) * si);
-        gstore(S.visc_i + gid, si * (lil * lil));
+        evi = si * (lil * lil);
     }
 }
 
+// The stress terms of point q (index i of its C-layer) from the eddy
+// viscosities at its C-node (evc), the C-node above (evn), and the I-nodes
+// below (evb) and above (eva), at clamped levels.
 template <typename T>
-__global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ spaces, LaunchTable lt) {
-    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
-    const Space<T> &S = spaces[E.space];
-    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
-    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
-    if (gid >= P * uint32_t(S.n3)) return;
-    const int l = int(gid / P);
-    const uint32_t x = gid - uint32_t(l) * P;
-    const int n3 = S.n3;
-    const T evc = gload(S.visc_c + gid);
-    const T evn = gload(S.visc_c + uint32_t(min(l + 1, n3 - 1)) * P + x);
-    const T evb = gload(S.visc_i + uint32_t(max(l - 1, 0)) * P + x);
-    const T eva = gload(S.visc_i + uint32_t(min(l, n3 - 2)) * P + x);
-    const SgsPoint<T> q(S, P, x, l);
+__device__ __forceinline__ void sgs_stress_point(const Space<T> &S, const SgsPoint<T> &q, uint32_t i, T evc, T evn,
+                                                 T evb, T eva) {
+    const int l = q.l, n3 = S.n3;
     // τ11, τ22, τ12 on C-nodes: rates vel1_1 -= τ11, vel2_2 -= τ22, vel1_2/vel2_1 -= τ12
-    gstore(S.src_phys[3] + gid, -(T(-2) * evc * q.u1[0]));
-    gstore(S.src_phys[4] + gid, -(T(-2) * evc * q.v2[0]));
-    gstore(S.src_phys[5] + gid, -(T(-2) * evc * q.s12(0)));
+    gstore(S.src_phys[3] + i, -(T(-2) * evc * q.u1[0]));
+    gstore(S.src_phys[4] + i, -(T(-2) * evc * q.v2[0]));
+    gstore(S.src_phys[5] + i, -(T(-2) * evc * q.s12(0)));
     // vel3 -= d/dx3 τ33 (dx3_c2i!) on I-layer l
     if (l < n3 - 1) {
         const T a = q.ai[1];
         const T t0 = T(-2) * evc * q.s33(0);
         const T t1 = T(-2) * evn * q.s33(1);
-        gstore(S.src_phys[2] + gid, -((-a) * t0 + a * t1));
-        gstore(S.src_phys[6] + gid, -(T(-2) * eva * q.s13(1)));
-        gstore(S.src_phys[7] + gid, -(T(-2) * eva * q.s23(1)));
+        gstore(S.src_phys[2] + i, -((-a) * t0 + a * t1));
+        gstore(S.src_phys[6] + i, -(T(-2) * eva * q.s13(1)));
+        gstore(S.src_phys[7] + i, -(T(-2) * eva * q.s23(1)));
     }
     // vel1 -= d/dx3 τ13, vel2 -= d/dx3 τ23 on C-layer l (dx3_i2c! with wall stress)
     const T ac = q.ac[0];
@@ -549,8 +552,47 @@ __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__
         } else {
             ta = T(0);
         }
-        gstore(S.src_phys[comp] + gid, -((-ac) * tb + ac * ta));
+        gstore(S.src_phys[comp] + i, -((-ac) * tb + ac * ta));
     }
+}
+
+// Eddy viscosity and stresses in one pass: a thread walks kSgsLevels
+// consecutive C-layers of one column x (columns fastest over the threads),
+// computing the viscosities of each layer and of the layer below the first
+// in registers; the stresses of a layer are formed once the viscosity of the
+// layer above it is known.  The C-node viscosity is also stored (read by the
+// mean profiles).
+template <typename T>
+__global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ spaces, LaunchTable lt) {
+    const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
+    const Space<T> &S = spaces[E.space];
+    const uint32_t P = uint32_t(S.N1) * S.N2;  // point counts < 2^31 (checked at set-up)
+    const uint32_t gid = uint32_t(blockIdx.x - E.block_begin) * blockDim.x + threadIdx.x;
+    const int n3 = S.n3;
+    const uint32_t nch = uint32_t(n3 + kSgsLevels - 1) / kSgsLevels;
+    if (gid >= P * nch) return;
+    const int c = int(gid / P);
+    const uint32_t x = gid - uint32_t(c) * P;
+    const int l0 = c * kSgsLevels, l1 = min(l0 + kSgsLevels, n3);
+    const int js = max(l0 - 1, 0), je = min(l1, n3 - 1);
+    // evc_p, evi_p: viscosities of layer j-1 (C-node, I-node above); evi_pp of layer j-2
+    T evc_p = T(0), evi_p = T(0), evi_pp = T(0);
+    SgsPoint<T> prev;
+    for (int j = js; j <= je; ++j) {
+        const SgsPoint<T> q(S, P, x, j);
+        T evc, evi;
+        sgs_visc_point(S, q, evc, evi);
+        if (j >= l0 && j < l1) gstore(S.visc_c + uint32_t(j) * P + x, evc);
+        const int l = j - 1;
+        if (l >= l0)  // layer l < n3-1: evn = evc(l+1), evb = evi(max(l-1, 0)), eva = evi(l)
+            sgs_stress_point(S, prev, uint32_t(l) * P + x, evc_p, evc, l > 0 ? evi_pp : evi_p, evi_p);
+        evi_pp = evi_p;
+        evi_p = evi;
+        evc_p = evc;
+        prev = q;
+    }
+    // the top layer: evn = its own C-node viscosity, evb = eva = evi(n3-2)
+    if (l1 == n3) sgs_stress_point(S, prev, uint32_t(n3 - 1) * P + x, evc_p, evc_p, evi_pp, evi_pp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1082,7 +1124,8 @@ __global__ void __launch_bounds__(256) k_stage_chunked(const ProjDesc<T> *__rest
                                                        int nmain) {
     extern __shared__ unsigned char smem_raw[];
     if (int(blockIdx.x) >= nmain) {
-        if constexpr (sizeof(T) != sizeof(double)) stage_chunked_body<double, CL>(descs00[blockIdx.x - nmain], 0, stage, dt, smem_raw);
+        if constexpr (sizeof(T) != sizeof(double))
+            stage_chunked_body<double, stage00_cl(CL)>(descs00[blockIdx.x - nmain], 0, stage, dt, smem_raw);
         return;
     }
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
@@ -1210,7 +1253,6 @@ void launch_physics(int kind, const Space<T> *spaces_dev, const LaunchTable &lt,
     if (nblocks <= 0) return;
     switch (kind) {
         case PHYS_ADV: k_adv_products<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
-        case PHYS_SGS_VISC: k_sgs_viscosity<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
         case PHYS_SGS_STRESS: k_sgs_stress<T><<<nblocks, 256, 0, st>>>(spaces_dev, lt); break;
     }
 }

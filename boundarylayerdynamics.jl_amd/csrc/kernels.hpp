@@ -18,8 +18,17 @@ constexpr int kC2r2dThreads = 1024;  // block of the fused r2c (one plane per bl
 enum PhysKind : int { PHYS_ADV = 0, PHYS_SGS_VISC = 1, PHYS_SGS_STRESS = 2, PHYS_SGS_MLP = 3, PHYS_SGS_MLP_STRESS = 4 };
 // C-node points per block of the learned-closure kernel (256 threads x chunks)
 constexpr int kMlpChunks = 4;
+// Chunk length of the float64 (0,0) column pass inside the chunked stage
+// launch of chunk length CL: short chunks keep that one-column pass from
+// raising the register count of the whole launch (n3 <= 64·CL levels, so
+// at most 256 chunks of this length, one thread each).
+__host__ __device__ constexpr int stage00_cl(int CL) { return CL <= 4 ? 1 : CL / 4; }
 // consecutive levels of one mode per thread of the linear-term kernel
 constexpr int kLinLevels = 4;
+// consecutive C-levels of one column per thread of the advection products
+constexpr int kColLevels = 8;
+// consecutive C-levels of one column per thread of the SGS eddy viscosity + stress kernel
+constexpr int kSgsLevels = 4;
 
 // Spectral linear terms of one model instance.
 template <typename T>

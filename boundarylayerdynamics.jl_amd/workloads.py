@@ -181,8 +181,9 @@ def algorithmic_bytes(cfg_sizes, kind, precision, path=None):
         if k == "adv":
             out["adv_products"] = P * br * ((3 * nC + 3 * nI) + (2 * nC + nI))
         else:
-            out["sgs_viscosity"] = P * br * ((6 * nC + 3 * nI) + (nC + nI))
-            out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (nC + nI) + (5 * nC + 3 * nI))
+            # eddy viscosity and stresses in one kernel: 9 terms read, 8 sources
+            # and the C-node viscosity written
+            out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (5 * nC + 3 * nI) + nC)
             # learned closure: 9 terms read, 6 stresses written
             out["sgs_mlp"] = P * br * ((6 * nC + 3 * nI) + 6 * nC)
     out["stage_project"] = (3 + 4 + 4) / 3 * S  # stage 1: u,du→utmp; stages 2,3: u,utmp,du→out
