@@ -24,9 +24,9 @@ constexpr int kMlpChunks = 4;
 // at most 256 chunks of this length, one thread each).
 __host__ __device__ constexpr int stage00_cl(int CL) { return CL <= 4 ? 1 : CL / 4; }
 // consecutive levels of one mode per thread of the linear-term kernel
-constexpr int kLinLevels = 4;
+constexpr int kLinLevels = 8;
 // consecutive C-levels of one column per thread of the advection products
-constexpr int kColLevels = 8;
+constexpr int kColLevels = 16;
 // consecutive C-levels of one column per thread of the SGS eddy viscosity + stress kernel
 constexpr int kSgsLevels = 4;
 
