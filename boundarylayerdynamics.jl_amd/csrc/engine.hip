@@ -1486,6 +1486,27 @@ class Engine : public EngineBase {
 
     // time of one call of f (µs): the fastest of three timed groups of reps
     // calls after a warm-up call, so that one disturbed group does not decide
+    // One right-hand side of the live state captured as a graph and replayed:
+    // microseconds per replay (best of time_us's groups).  The transform path
+    // is chosen on this figure, the way the steps run, rather than on eager
+    // launches, whose cross-stream waits and dispatch order differ.
+    double time_rhs_graph(int reps) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        const bool prof = profiling;
+        profiling = false;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        rhs(0);
+        HIPCHECK(hipStreamEndCapture(st, &g));
+        profiling = prof;
+        HIPCHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        HIPCHECK(hipGraphDestroy(g));
+        const double us = time_us([&] { HIPCHECK(hipGraphLaunch(ge, st)); }, reps);
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGraphExecDestroy(ge));
+        return us;
+    }
+
     template <typename F>
     double time_us(F f, int reps) {
         f();
@@ -1553,7 +1574,7 @@ class Engine : public EngineBase {
                 fused_r2c = (mode & 2) != 0;
                 build_transform_passes(&best);
                 if (mode < 3 && !any_fused_space()) break;  // nothing qualifies: one choice only
-                const double us = time_us([&] { rhs(0); }, reps);
+                const double us = time_rhs_graph(std::max(reps, 10));
                 if (us < bt) {
                     bt = us;
                     bm = mode;
@@ -1566,15 +1587,15 @@ class Engine : public EngineBase {
             bool any_half = false;
             for (auto &g : tp_c2r2d) any_half |= !g[2].empty();
             if (any_half) {
-                // whole-plane blocks stay unless half planes are clearly faster
-                // (≥ 3 %), so that the choice does not follow timing noise
+                // whole-plane blocks stay unless half planes are faster by
+                // more than 1 %, so that the choice does not follow timing noise
                 double us_h[2];
                 for (int h = 1; h >= 0; --h) {
                     half_c2r = h == 1;
                     build_transform_passes(&best);
-                    us_h[h] = time_us([&] { rhs(0); }, reps);
+                    us_h[h] = time_rhs_graph(std::max(reps, 10));
                 }
-                const int bh = us_h[1] < 0.97 * us_h[0] ? 1 : 0;
+                const int bh = us_h[1] < 0.99 * us_h[0] ? 1 : 0;
                 half_c2r = bh == 1;
                 bt = us_h[bh];
                 bm = (bm & 3) | (bh << 2);
