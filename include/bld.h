@@ -127,7 +127,7 @@ int bld_autotune(bld_engine *e, int reps);
  * of the fused stage kernel and the time of one stage; pass 5: the transform
  * path (bit 0: whole-plane c2r, bit 1: whole-plane r2c, for the square grids
  * that fit in LDS; bit 2: c2r of the largest planes by half planes) and the
- * time of one right-hand side (entry ignored). */
+ * time of one graph-replayed SSPRK33 step (entry ignored). */
 int bld_tuning(bld_engine *e, int pass, int entry, int *batch, double *us);
 /* Transform path forced for testing and measurement (no reference
  * counterpart): the pass-5 bits of bld_tuning (bit 0 whole-plane c2r, bit 1
