@@ -297,58 +297,104 @@ __device__ __forceinline__ void c2r_2h_reg(const Space<T> &S, const LaunchEntry 
     cplx<T> *Z = smem;
     cplx<T> *TW = smem + size_t(M) * zs;
     const size_t P2 = size_t(K1) * K2;
-    for (int j = threadIdx.x; j < N; j += NT) TW[j] = gload(S.tw + S.p2.tw + j);
+    // the twiddle table, loaded together with the first round's gathers and
+    // stored to LDS behind them
+    constexpr int NTW = (N + NT - 1) / NT;
+    cplx<T> twv[NTW];
+    static_for<0, NTW>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int e = j * NT + int(threadIdx.x);
+        twv[j] = e < N ? gload(S.tw + S.p2.tw + e) : czero<T>();
+    });
     const cplx<T> *u = S.state[sel][0], *vv = S.state[sel][1], *w = S.state[sel][2];
-    const cplx<T> *__restrict__ tw = S.tw + S.p2.tw;  // w_N^j; w_M^j = w_N^{2j}
 
-    // x2, stage 1: item (c, t); AM values Y_r[k] at k = BM·n + t
-    for (int it = threadIdx.x; it < nk1 * BM; it += NT) {
-        const int t = it / nk1, c = it - t * nk1;
-        const size_t base = size_t(layer) * P2 + c;
-        cplx<T> v[AM];
+    // x2, stage 1: item (c, t), c fastest; AM values Y_r[k] at k = BM·n + t,
+    // in rounds of NT items (the same count for every thread)
+    {
+        const int items = nk1 * BM, rounds = (items + NT - 1) / NT;
         const T sg = r ? T(-1) : T(1);
-        // resolved index of k2 position p (clamped to 0 where p is padding)
-        auto idx = [&](int p, bool &ok) {
-            const bool lo = p <= k2m, hi = p >= N - k2m;
-            ok = lo || hi;
-            return lo ? p : (hi ? K2 - (N - p) : 0);
-        };
-        auto gather = [&](auto value) {
-            static_for<0, AM>([&](auto nc) {
-                constexpr int n = decltype(nc)::value;
-                bool oka, okb;
-                const int ia = idx(BM * n + t, oka), ib = idx(BM * n + t + M, okb);
-                const cplx<T> xa = value(base + size_t(K1) * ia, ia);
-                const cplx<T> xb = value(base + size_t(K1) * ib, ib);
-                const cplx<T> a = oka ? xa : czero<T>(), b = okb ? xb : czero<T>();
-                v[n] = mk(a.x + sg * b.x, a.y + sg * b.y);
-            });
-        };
-        const T d1 = T(gload(S.D1 + c));
-        switch (recipe) {  // uniform per block
-            case R_U: gather([&](size_t i, int) { return gload(u + i); }); break;
-            case R_V: gather([&](size_t i, int) { return gload(vv + i); }); break;
-            case R_W: gather([&](size_t i, int) { return gload(w + i); }); break;
-            case R_D1U: gather([&](size_t i, int) { return imul(d1, gload(u + i)); }); break;
-            case R_D1V: gather([&](size_t i, int) { return imul(d1, gload(vv + i)); }); break;
-            case R_D1W: gather([&](size_t i, int) { return imul(d1, gload(w + i)); }); break;
-            case R_D2U: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(u + i)); }); break;
-            case R_D2V: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(vv + i)); }); break;
-            case R_D2W: gather([&](size_t i, int i2) { return imul(T(gload(S.D2 + i2)), gload(w + i)); }); break;
-            default: gather([&](size_t i, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
+        for (int rd = 0; rd < rounds; ++rd) {
+            const int it = rd * NT + int(threadIdx.x);
+            const bool have = it < items;
+            const int t = (have ? it : 0) / nk1, c = (have ? it : 0) - t * nk1;
+            const uint32_t base = uint32_t(layer) * uint32_t(P2) + uint32_t(c);
+            cplx<T> v[AM];
+            // resolved index of k2 position p (clamped to 0 where p is padding)
+            auto idx = [&](int p, bool &ok) {
+                const bool lo = p <= k2m, hi = p >= N - k2m;
+                ok = lo || hi;
+                return lo ? p : (hi ? K2 - (N - p) : 0);
+            };
+            // padding positions load nothing: with 64 columns per wave their
+            // test is uniform across the wave
+            auto gather = [&](auto value) {
+                static_for<0, AM>([&](auto nc) {
+                    constexpr int n = decltype(nc)::value;
+                    bool oka, okb;
+                    const int ia = idx(BM * n + t, oka), ib = idx(BM * n + t + M, okb);
+                    cplx<T> a = czero<T>(), b = czero<T>();
+                    if (have && oka) a = value(base + uint32_t(K1) * uint32_t(ia), ia);
+                    if (have && okb) b = value(base + uint32_t(K1) * uint32_t(ib), ib);
+                    v[n] = mk(a.x + sg * b.x, a.y + sg * b.y);
+                });
+            };
+            const T d1 = T(gload(S.D1 + c));
+            auto d2 = [&](int i2) { return T(gload(S.D2 + i2)); };
+            switch (recipe) {  // uniform per block
+                case R_U: gather([&](uint32_t i, int) { return gload(u + i); }); break;
+                case R_V: gather([&](uint32_t i, int) { return gload(vv + i); }); break;
+                case R_W: gather([&](uint32_t i, int) { return gload(w + i); }); break;
+                case R_D1U: gather([&](uint32_t i, int) { return imul(d1, gload(u + i)); }); break;
+                case R_D1V: gather([&](uint32_t i, int) { return imul(d1, gload(vv + i)); }); break;
+                case R_D1W: gather([&](uint32_t i, int) { return imul(d1, gload(w + i)); }); break;
+                case R_D2U: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(u + i)); }); break;
+                case R_D2V: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(vv + i)); }); break;
+                case R_D2W: gather([&](uint32_t i, int i2) { return imul(d2(i2), gload(w + i)); }); break;
+                // vorticity (derivatives.jl:566-568), the operations of term_value
+                case R_VORT1: {
+                    const T al = T(gload(S.alpha_i + layer + 1));
+                    gather([&](uint32_t i, int i2) {
+                        const cplx<T> dz = (-al) * gload(vv + i) + al * gload(vv + i + uint32_t(P2));
+                        return imul(d2(i2), gload(w + i)) - dz;
+                    });
+                    break;
+                }
+                case R_VORT2: {
+                    const T al = T(gload(S.alpha_i + layer + 1));
+                    gather([&](uint32_t i, int) {
+                        const cplx<T> dz = (-al) * gload(u + i) + al * gload(u + i + uint32_t(P2));
+                        return dz - imul(d1, gload(w + i));
+                    });
+                    break;
+                }
+                case R_VORT3:
+                    gather([&](uint32_t i, int i2) { return imul(d1, gload(vv + i)) - imul(d2(i2), gload(u + i)); });
+                    break;
+                default: gather([&](uint32_t, int i2) { return term_value(S, term, recipe, sel, layer, c, i2); });
+            }
+            if (rd == 0) {  // the twiddles in LDS for every round
+                static_for<0, NTW>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value;
+                    const int e = j * NT + int(threadIdx.x);
+                    if (e < N) TW[e] = twv[j];
+                });
+                __syncthreads();
+            }
+            if (have) {
+                if (r) {  // e^{2πi·k/N}
+                    static_for<1, AM>([&](auto nc) {
+                        constexpr int n = decltype(nc)::value;
+                        v[n] = cmul(v[n], ltwiddle<T, true>(TW, BM * n + t));
+                    });
+                    if (t) v[0] = cmul(v[0], ltwiddle<T, true>(TW, t));
+                }
+                rdft<T, true, AM>(v);
+                static_for<0, AM>([&](auto kc) {
+                    constexpr int k = decltype(kc)::value;
+                    Z[(t * AM + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], ltwiddle<T, true>(TW, 2 * ((t * k) % M)));
+                });
+            }
         }
-        if (r) {  // e^{2πi·k/N}
-            static_for<1, AM>([&](auto nc) {
-                constexpr int n = decltype(nc)::value;
-                v[n] = cmul(v[n], gtwiddle<T, true>(tw, BM * n + t));
-            });
-            if (t) v[0] = cmul(v[0], gtwiddle<T, true>(tw, t));
-        }
-        rdft<T, true, AM>(v);
-        static_for<0, AM>([&](auto kc) {
-            constexpr int k = decltype(kc)::value;
-            Z[(t * AM + k) * zs + c] = k == 0 ? v[0] : cmul(v[k], gtwiddle<T, true>(tw, 2 * ((t * k) % M)));
-        });
     }
     __syncthreads();
     // x2, stage 2 in place: local rows t + AM·k
