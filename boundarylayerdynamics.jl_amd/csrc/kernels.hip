@@ -450,6 +450,26 @@ struct SgsPoint {
         ac[0] = T(gload(s.alpha_c + l)), ac[1] = T(gload(s.alpha_c + min(l + 1, n3 - 1)));
         ai[0] = T(gload(s.alpha_i + l)), ai[1] = T(gload(s.alpha_i + l + 1));
     }
+    // the point of the C-layer above (l + 1 <= n3 - 1): the values it shares
+    // with this one are moved, the others loaded — the same values as a
+    // point constructed there
+    __device__ void advance(uint32_t P, uint32_t x) {
+        ++l;
+        const uint32_t cp = uint32_t(min(l + 1, n3 - 1)) * P + x, ip = uint32_t(min(l + 1, n3 - 2)) * P + x;
+        const uint32_t i0 = uint32_t(min(l, n3 - 2)) * P + x;
+        const T *const *f = S->in_phys;
+        u[0] = u[1], u[1] = u[2], u[2] = gload(f[0] + cp);
+        v[0] = v[1], v[1] = v[2], v[2] = gload(f[1] + cp);
+        w[0] = w[1], w[1] = w[2], w[2] = gload(f[2] + ip);
+        u1[0] = u1[1], u1[1] = gload(f[3] + cp);
+        v2[0] = v2[1], v2[1] = gload(f[4] + cp);
+        u2[0] = u2[1], u2[1] = gload(f[5] + cp);
+        v1[0] = v1[1], v1[1] = gload(f[6] + cp);
+        w1[0] = w1[1], w1[1] = gload(f[7] + i0);
+        w2[0] = w2[1], w2[1] = gload(f[8] + i0);
+        ac[0] = ac[1], ac[1] = T(gload(S->alpha_c + min(l + 1, n3 - 1)));
+        ai[0] = ai[1], ai[1] = T(gload(S->alpha_i + l + 1));
+    }
     // S33 on C-layer l (k = 0) or l+1 (k = 1) (vel3_3 via dx3_i2c!, physical Dirichlet values)
     __device__ T s33(int k) const {
         const int lk = l + k;
@@ -577,9 +597,9 @@ __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__
     const int js = max(l0 - 1, 0), je = min(l1, n3 - 1);
     // evc_p, evi_p: viscosities of layer j-1 (C-node, I-node above); evi_pp of layer j-2
     T evc_p = T(0), evi_p = T(0), evi_pp = T(0);
-    SgsPoint<T> prev;
+    SgsPoint<T> prev, q(S, P, x, js);
     for (int j = js; j <= je; ++j) {
-        const SgsPoint<T> q(S, P, x, j);
+        if (j > js) q.advance(P, x);
         T evc, evi;
         sgs_visc_point(S, q, evc, evi);
         if (j >= l0 && j < l1) gstore(S.visc_c + uint32_t(j) * P + x, evc);
