@@ -22,7 +22,9 @@ BLD_SSPRK33, BLD_SSPRK22, BLD_EULER, BLD_AB2 = 0, 1, 2, 3
 PROFILES = ("vel1", "vel2", "vel3", "adv1", "adv2", "adv3", "sgs11", "sgs12", "sgs13", "sgs22", "sgs23", "sgs33")
 I_PROFILES = ("vel3", "adv3", "sgs13", "sgs23")
 
-# kernel classes reported by bld_kernel_times (order of the engine's KernelKind)
+# kernel classes reported by bld_kernel_times (order of the engine's KernelKind;
+# "sgs_viscosity" has no launches: the eddy viscosity is computed inside the
+# "sgs_stress" kernel)
 KERNEL_KINDS = ("linear", "c2r_x2", "c2r_x1", "adv_products", "sgs_viscosity", "sgs_stress", "r2c_x1",
                 "r2c_x2", "stage_project", "check_finite", "sgs_mlp", "c2r_2d", "r2c_2d")
 
