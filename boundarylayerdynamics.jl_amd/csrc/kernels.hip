@@ -431,7 +431,8 @@ struct SgsPoint {
     T w[3];                 // I-levels l-1, l, l+1
     T u1[2], v2[2], u2[2], v1[2];  // C-levels l, l+1
     T w1[2], w2[2];         // I-levels l-1, l
-    T ac[2], ai[2];         // alpha_c[l], alpha_c[l+1]; alpha_i[l], alpha_i[l+1]
+    T ac[2], ai[2];         // alpha_c[l], alpha_c[l+1]; alpha_i[l], alpha_i[l+1] (set by the walk)
+    T lc, li;               // filter widths of C-layer l and I-layer min(l, n3-2) (set by the walk)
     SgsPoint() = default;
     __device__ SgsPoint(const Space<T> &s, uint32_t P, uint32_t x, int lev) : S(&s), l(lev), n3(s.n3) {
         const uint32_t cm = uint32_t(max(l - 1, 0)) * P + x, c0 = uint32_t(l) * P + x;
@@ -447,28 +448,38 @@ struct SgsPoint {
         v1[0] = gload(f[6] + c0), v1[1] = gload(f[6] + cp);
         w1[0] = gload(f[7] + cm), w1[1] = gload(f[7] + i0);
         w2[0] = gload(f[8] + cm), w2[1] = gload(f[8] + i0);
-        ac[0] = T(gload(s.alpha_c + l)), ac[1] = T(gload(s.alpha_c + min(l + 1, n3 - 1)));
-        ai[0] = T(gload(s.alpha_i + l)), ai[1] = T(gload(s.alpha_i + l + 1));
     }
-    // the point of the C-layer above (l + 1 <= n3 - 1): the values it shares
-    // with this one are moved, the others loaded — the same values as a
-    // point constructed there
-    __device__ void advance(uint32_t P, uint32_t x) {
-        ++l;
-        const uint32_t cp = uint32_t(min(l + 1, n3 - 1)) * P + x, ip = uint32_t(min(l + 1, n3 - 2)) * P + x;
-        const uint32_t i0 = uint32_t(min(l, n3 - 2)) * P + x;
+    // The field values that the point of C-layer m takes in on top of those
+    // it shares with the point of layer m - 1 (m clamped to the top layer),
+    // loaded ahead of their use so that they are in flight while the layer
+    // below is being formed.
+    struct Next {
+        T u, v, w, u1, v2, u2, v1, w1, w2;
+    };
+    __device__ Next fetch(uint32_t P, uint32_t x, int m) const {
+        m = min(m, n3 - 1);
+        const uint32_t cp = uint32_t(min(m + 1, n3 - 1)) * P + x, ip = uint32_t(min(m + 1, n3 - 2)) * P + x;
+        const uint32_t i0 = uint32_t(min(m, n3 - 2)) * P + x;
         const T *const *f = S->in_phys;
-        u[0] = u[1], u[1] = u[2], u[2] = gload(f[0] + cp);
-        v[0] = v[1], v[1] = v[2], v[2] = gload(f[1] + cp);
-        w[0] = w[1], w[1] = w[2], w[2] = gload(f[2] + ip);
-        u1[0] = u1[1], u1[1] = gload(f[3] + cp);
-        v2[0] = v2[1], v2[1] = gload(f[4] + cp);
-        u2[0] = u2[1], u2[1] = gload(f[5] + cp);
-        v1[0] = v1[1], v1[1] = gload(f[6] + cp);
-        w1[0] = w1[1], w1[1] = gload(f[7] + i0);
-        w2[0] = w2[1], w2[1] = gload(f[8] + i0);
-        ac[0] = ac[1], ac[1] = T(gload(S->alpha_c + min(l + 1, n3 - 1)));
-        ai[0] = ai[1], ai[1] = T(gload(S->alpha_i + l + 1));
+        Next n;
+        n.u = gload(f[0] + cp), n.v = gload(f[1] + cp), n.w = gload(f[2] + ip);
+        n.u1 = gload(f[3] + cp), n.v2 = gload(f[4] + cp), n.u2 = gload(f[5] + cp), n.v1 = gload(f[6] + cp);
+        n.w1 = gload(f[7] + i0), n.w2 = gload(f[8] + i0);
+        return n;
+    }
+    // the point of the C-layer above (l + 1 <= n3 - 1) from the values fetched
+    // for it: the same values as a point constructed there
+    __device__ void advance(const Next &n) {
+        ++l;
+        u[0] = u[1], u[1] = u[2], u[2] = n.u;
+        v[0] = v[1], v[1] = v[2], v[2] = n.v;
+        w[0] = w[1], w[1] = w[2], w[2] = n.w;
+        u1[0] = u1[1], u1[1] = n.u1;
+        v2[0] = v2[1], v2[1] = n.v2;
+        u2[0] = u2[1], u2[1] = n.u2;
+        v1[0] = v1[1], v1[1] = n.v1;
+        w1[0] = w1[1], w1[1] = n.w1;
+        w2[0] = w2[1], w2[1] = n.w2;
     }
     // S33 on C-layer l (k = 0) or l+1 (k = 1) (vel3_3 via dx3_i2c!, physical Dirichlet values)
     __device__ T s33(int k) const {
@@ -488,7 +499,7 @@ struct SgsPoint {
 template <typename T>
 __device__ __forceinline__ void sgs_visc_point(const Space<T> &S, const SgsPoint<T> &q, T &evc, T &evi) {
     const int l = q.l, n3 = S.n3;
-    const T lcl = T(gload(S.lc + l)), lil = T(gload(S.li + min(l, n3 - 2)));
+    const T lcl = q.lc, lil = q.li;
     // C-nodes: total_strain! (:208, :225-229)
     const T S11 = q.u1[0], S22 = q.v2[0], S33 = q.s33(0), S12 = q.s12(0);
     T sc = S11 * S11 + S22 * S22 + S33 * S33 + T(2) * S12 * S12;
@@ -597,9 +608,32 @@ __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__
     const int js = max(l0 - 1, 0), je = min(l1, n3 - 1);
     // evc_p, evi_p: viscosities of layer j-1 (C-node, I-node above); evi_pp of layer j-2
     T evc_p = T(0), evi_p = T(0), evi_pp = T(0);
+    // layer factors of the walk (at clamped levels), loaded before the fields
+    // so that converting them waits for them alone
+    constexpr int NW = kSgsLevels + 2;  // layers of the walk at most
+    T fac[NW + 1], fai[NW + 1], flc[NW], fli[NW];
+#pragma unroll
+    for (int t = 0; t <= NW; ++t) {
+        fac[t] = T(gload(S.alpha_c + min(js + t, n3 - 1)));
+        fai[t] = T(gload(S.alpha_i + min(js + t, n3)));
+        if (t < NW) {
+            flc[t] = T(gload(S.lc + min(js + t, n3 - 1)));
+            fli[t] = T(gload(S.li + min(js + t, n3 - 2)));
+        }
+    }
+    // the walk is unrolled over the chunk and the loads of layer j+1 are issued
+    // before layer j is formed (unconditionally, at a clamped layer), so one
+    // layer's loads are always in flight, without waits at joins or on copies
     SgsPoint<T> prev, q(S, P, x, js);
-    for (int j = js; j <= je; ++j) {
-        if (j > js) q.advance(P, x);
+    typename SgsPoint<T>::Next nx[2];
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int j = js + t;
+        if (j > je) break;
+        nx[t & 1] = q.fetch(P, x, j + 1);
+        if (t > 0) q.advance(nx[(t - 1) & 1]);
+        q.ac[0] = fac[t], q.ac[1] = fac[t + 1], q.ai[0] = fai[t], q.ai[1] = fai[t + 1];
+        q.lc = flc[t], q.li = fli[t];
         T evc, evi;
         sgs_visc_point(S, q, evc, evi);
         if (j >= l0 && j < l1) gstore(S.visc_c + uint32_t(j) * P + x, evc);
