@@ -287,8 +287,19 @@ __device__ __forceinline__ void c2r_2h_reg(const Space<T> &S, const LaunchEntry 
     constexpr int ROUNDS = (NP * Bq + NT - 1) / NT;
     constexpr int P = (NP + ROUNDS - 1) / ROUNDS;
     static_assert(P * Bq <= NT, "one first-stage row item per thread");
+    // Blocks are dealt to the 8 XCDs in turn, so blocks 8 apart share an L2:
+    // the two halves of a plane are blocks 8h + x of a group of 16 (half h),
+    // and the second half's gathers hit what the first half brought in.
     const int jb = blockIdx.x - E.block_begin;
-    const int job = jb >> 1, r = jb & 1;
+    const int nb16 = (2 * E.njobs) & ~15;
+    int job, r;
+    if (jb < nb16) {
+        job = (jb >> 4) * 8 + (jb & 7);
+        r = (jb >> 3) & 1;
+    } else {
+        job = (nb16 >> 1) + ((jb - nb16) >> 1);
+        r = (jb - nb16) & 1;
+    }
     int term = 0;
     while (job >= S.in_job0[term + 1]) ++term;
     const int layer = job - S.in_job0[term];
