@@ -711,10 +711,14 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
     const int l0 = c * kLinLevels;
     const cplx<T> *q = L.state[sel][f];
     cplx<T> *du = L.du[f];
-    if (m == 0 && L.shadow[sel][f] != nullptr) {
-        // (0,0) mode of the mixed form: float64 from the float64 shadow
-        const cplx<double> *qs = L.shadow[sel][f];
-        for (int l = l0; l < min(l0 + kLinLevels, nl); ++l) {
+    const cplx<double> *qs = L.shadow[sel][f];
+    if (qs != nullptr && m < kLinLevels) {
+        // (0,0) mode of the mixed form, float64 from the float64 shadow: one
+        // level per thread of the chunk's first modes (mode m takes levels
+        // l0 + m, l0 + m + nk, ...), so that its levels run side by side
+        for (int j = m; j < kLinLevels; j += nk) {
+            const int l = l0 + j;
+            if (l >= nl) break;
             cplx<double> res = mk(0.0, 0.0);
             if (L.has_diff[f]) {
                 cplx<double> lo, hi, cc = qs[l];
@@ -726,14 +730,14 @@ __global__ void __launch_bounds__(256) k_linear(const LinDesc<T> *__restrict__ d
                     lo = l > 0 ? qs[l - 1] : mk(L.bc_val[2][0], 0.0);
                     hi = l < nl - 1 ? qs[l + 1] : mk(L.bc_val[2][1], 0.0);
                 }
-                res = laplacian_point<T, double>(L, f, l, m, cc, lo, hi);
+                res = laplacian_point<T, double>(L, f, l, 0, cc, lo, hi);
             }
             res.x += L.src[f];
             L.du00[f][l] = res;
             du[size_t(l) * nk] = mk(T(0), T(0));
         }
-        return;
     }
+    if (m == 0 && qs != nullptr) return;
     const bool diff = L.has_diff[f] != 0;
     const T src = T(L.src[f]);
     // the column window: levels l0-1 .. l0+kLinLevels at clamped indices, and
