@@ -8,7 +8,7 @@ forcing) is timed against the plain float64 form on one GPU.  A step is one
 SSPRK33 step (3 right-hand sides, 3 fused stage+projection passes), with the
 state resident in HBM; steps are replayed from a captured HIP graph.
 
-Prints ONE JSON line (see README of the metric in BASELINE.json).
+Prints ONE JSON line: steps/s of the chosen form, with the other legs as fields.
 """
 
 import argparse
@@ -437,7 +437,7 @@ PROFILE_NAMES = ("vel1", "vel2", "sgs11", "sgs22", "sgs33", "sgs12", "sgs13", "s
 
 
 def closure_vs_smagorinsky(B, params, res, dt, steps):
-    """The LES with the learned closure judged against the same LES with
+    """The LES with the learned closure compared with the same LES with
     Smagorinsky: both from the same seeded state, `steps` SSPRK33 steps with
     the mean profiles of the reference's MeanProfiles collected on the device;
     the relative L2 difference of the time-mean profiles (velocity and the
@@ -728,7 +728,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine has no multi-GPU path "
-              f"in this build", file=sys.stderr)
+              f"in this library", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()

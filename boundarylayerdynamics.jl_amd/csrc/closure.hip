@@ -123,7 +123,8 @@ __global__ void __launch_bounds__(256) k_sgs_mlp(const Space<T> *__restrict__ sp
                 t[0] -= tr;
                 t[1] -= tr;
                 t[2] -= tr;
-                // dissipation guard: no backscatter (−τ:S < 0) where Smagorinsky judges
+                // dissipation guard: where the learned stress would backscatter (−τ:S < 0),
+                // the Smagorinsky stress is used instead
                 if (S.mlp_guard) {
                     const float eps = -(t[0] * s11 + t[1] * s22 + t[2] * s33 +
                                         2.f * (t[3] * s12 + t[4] * s13 + t[5] * s23));

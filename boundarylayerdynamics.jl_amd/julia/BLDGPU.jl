@@ -8,7 +8,7 @@ BoundaryLayerDynamics.jl.  `BLDGPU.evolve!(model, tspan; dt)` is a drop-in for
 discretised processes are handed to the engine once, the state is uploaded,
 all steps run on the GPU, and the state is written back into `model.state`.
 
-Not executed in the build environment (no Julia toolchain); see INTEGRATION.md.
+Not executed in the build environment (no Julia toolchain).
 """
 module BLDGPU
 

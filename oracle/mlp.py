@@ -1,6 +1,6 @@
 """CPU oracle of the learned SGS closure network (test infrastructure only).
 
-The closure has no counterpart in the reference: it is a north-star addition
+The closure has no counterpart in the reference: it is an addition
 that sits behind the reference's process API next to StaticSmagorinskyModel
 (static_smagorinsky.jl:39-42).  This module restates the network the GPU
 kernels evaluate, in float64 NumPy:

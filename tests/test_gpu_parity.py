@@ -1,7 +1,7 @@
 """Parity of the HIP engine (through the C ABI) with the CPU oracle.
 
 Float64 plain form: rates, projections and multi-step runs must agree with the
-oracle to 1e-10 relative; single right-hand sides agree
+oracle to 1e-10 relative (the parity bar); single right-hand sides agree
 to ~1e-13.  Float32 mixed form: 2e-5 relative for one right-hand side.
 """
 
