@@ -1124,6 +1124,14 @@ class Engine : public EngineBase {
         I.prof_mask = mask;
         I.prof_steps = 0;
         drop_graph();
+        if (I.sgs >= 0) {  // the τ33 profile reads the stored C-node eddy viscosity
+            const int sv = (mask >> PROF_SGS33) & 1;
+            if (spaces[I.sgs].store_visc != sv) {
+                spaces[I.sgs].store_visc = sv;
+                HIPCHECK(hipDeviceSynchronize());
+                sync_spaces();
+            }
+        }
         if (mask) {  // first sample of the current state (the rate is evaluated first)
             rhs(0);
             sample_profiles(0.0);

@@ -591,8 +591,8 @@ __device__ __forceinline__ void sgs_stress_point(const Space<T> &S, const SgsPoi
 // consecutive C-layers of one column x (columns fastest over the threads),
 // computing the viscosities of each layer and of the layer below the first
 // in registers; the stresses of a layer are formed once the viscosity of the
-// layer above it is known.  The C-node viscosity is also stored (read by the
-// mean profiles).
+// layer above it is known.  The C-node viscosity is also stored while the τ33
+// mean profile, which reads it, is collected.
 template <typename T>
 __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__ spaces, LaunchTable lt) {
     const LaunchEntry E = lt.e[find_entry(lt, blockIdx.x)];
@@ -636,7 +636,7 @@ __global__ void __launch_bounds__(256) k_sgs_stress(const Space<T> *__restrict__
         q.lc = flc[t], q.li = fli[t];
         T evc, evi;
         sgs_visc_point(S, q, evc, evi);
-        if (j >= l0 && j < l1) gstore(S.visc_c + uint32_t(j) * P + x, evc);
+        if (S.store_visc && j >= l0 && j < l1) gstore(S.visc_c + uint32_t(j) * P + x, evc);
         const int l = j - 1;
         if (l >= l0)  // layer l < n3-1: evn = evc(l+1), evb = evi(max(l-1, 0)), eva = evi(l)
             sgs_stress_point(S, prev, uint32_t(l) * P + x, evc_p, evc, l > 0 ? evi_pp : evi_p, evi_p);

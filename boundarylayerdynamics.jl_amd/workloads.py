@@ -182,8 +182,8 @@ def algorithmic_bytes(cfg_sizes, kind, precision, path=None):
             out["adv_products"] = P * br * ((3 * nC + 3 * nI) + (2 * nC + nI))
         else:
             # eddy viscosity and stresses in one kernel: 9 terms read, 8 sources
-            # and the C-node viscosity written
-            out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (5 * nC + 3 * nI) + nC)
+            # written (the C-node viscosity only while the τ33 profile is on)
+            out["sgs_stress"] = P * br * ((6 * nC + 3 * nI) + (5 * nC + 3 * nI))
             # learned closure: 9 terms read, 6 stresses written
             out["sgs_mlp"] = P * br * ((6 * nC + 3 * nI) + 6 * nC)
     out["stage_project"] = (3 + 4 + 4) / 3 * S  # stage 1: u,du→utmp; stages 2,3: u,utmp,du→out
