@@ -123,7 +123,7 @@ struct Space {
     const double *D1, *D2;
     const double *alpha_c, *alpha_i;  // [n3], [n3+1]
     // physical-process data
-    T *visc_c, *visc_i;               // SGS eddy viscosity
+    T *visc_c;                        // SGS eddy viscosity at the C-nodes
     int store_visc;                   // the SGS kernel stores visc_c (read by the τ33 mean profile only)
     const double *lc, *li;            // SGS length scales
     int wall_rough[2];

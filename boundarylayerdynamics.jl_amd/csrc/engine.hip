@@ -619,7 +619,6 @@ class Engine : public EngineBase {
             S.tgt_nG[2] = 2; S.tgt_G[2][0] = 2; S.tgt_Gd1[2][0] = 0; S.tgt_G[2][1] = 6; S.tgt_Gd1[2][1] = 1;
             S.tgt_nH[2] = 1; S.tgt_H[2][0] = 7;
             S.visc_c = I.own(P * n3 * sizeof(T)).template as<T>();
-            S.visc_i = I.own(P * (n3 - 1) * sizeof(T)).template as<T>();
             S.lc = I.d_lc.template as<double>();
             S.li = I.d_li.template as<double>();
             for (int s = 0; s < 2; ++s) {
