@@ -209,3 +209,36 @@ def test_c2_les_full_size_fp32():
         e_max, e_l2 = relerr(got[f], ref.state[f]), rel_l2(got[f], ref.state[f])
         print(f"fp32 C2 {f}: max {e_max:.3e} l2 {e_l2:.3e}")
         assert e_max < RTOL_STEPS32_MAX and e_l2 < RTOL_STEPS32_L2, (f, e_max, e_l2)
+
+
+def test_ensemble_bench_classes_fp32():
+    """The uneven ensemble's transform classes (advection planes of 72², 96²,
+    144² and 192² from the 2^a·3^b dealiasing of the ensemble benchmark, SGS
+    planes of 48² to 128²) in one autotuned fp32 ensemble, a few levels each:
+    every member against the oracle after 2 SSPRK33 steps, within the float32
+    bounds of the headline tests."""
+    grids = [(48, 48, 4), (64, 64, 4), (96, 96, 4), (128, 128, 4)]
+    cs = [0.08, 0.11, 0.13, 0.16]
+    refs, specs, states = [], [], []
+    for i, (g, c) in enumerate(zip(grids, cs)):
+        ref = O.Model(*W.les_neutral(O, g, Cs=c, fft_friendly=True))
+        st = broadband_state(ref, 300 + i)
+        for f in ref.fields:
+            ref.state[f][...] = st[f]
+        refs.append(ref)
+        states.append(st)
+        specs.append(W.les_neutral(B, g, Cs=c, fft_friendly=True))
+    assert [tuple(s[2][0].dealiasing) for s in specs] == [(72, 72), (96, 96), (144, 144), (192, 192)]
+    ens = B.Ensemble(specs, precision="fp32")
+    for m, st in zip(ens.members, states):
+        m.set_state(**st)
+    ens.init_problem()
+    ens.autotune(reps=2)
+    dt = 5e-4
+    ens.step(dt, 2)
+    for ref, m, g in zip(refs, ens.members, grids):
+        O.evolve(ref, 2 * dt, dt)
+        got = m.state
+        for f in ref.fields:
+            e_max, e_l2 = relerr(got[f], ref.state[f]), rel_l2(got[f], ref.state[f])
+            assert e_max < RTOL_STEPS32_MAX and e_l2 < RTOL_STEPS32_L2, (g, f, e_max, e_l2)
