@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) k_
 #undef BLD_CALL
 }
 
-// half-plane c2r: blocks 2j and 2j+1 produce the even and the odd rows of plane j
+// half-plane c2r: two blocks per plane, one for its even and one for its odd
+// rows (placed 8 blocks apart, on one XCD)
 template <typename T>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
 k_c2r_2h(const Space<T> *__restrict__ spaces, LaunchTable lt, int sel) {
