@@ -83,6 +83,14 @@ CASES = {
     # stretched grid, free slip both sides
     "stretched": dict(res=(8, 12, 10), dims=(1.0, 2.0, 1.0), lower="SmoothWall", upper="SmoothWall",
                       mapping=0.7, procs=lambda M: M.incompressible_flow(0.05, constant_forcing=2)),
+    # the fewest vertical layers the grid allows (Grids.jl:28): every boundary
+    # stencil reaches across the whole column
+    "les_min": dict(res=(8, 8, 3), dims=(2 * np.pi, 2 * np.pi, 1.0), lower=("RoughWall", 1e-4),
+                    upper="FreeSlipBoundary",
+                    procs=lambda M: M.incompressible_flow(1e-5, constant_forcing=1,
+                                                          sgs_model=M.StaticSmagorinskyModel(Cs=0.1, wall_damping=True))),
+    "channel_min": dict(res=(6, 8, 3), dims=(2 * np.pi, np.pi, 2.0), lower="SmoothWall", upper="SmoothWall",
+                        procs=lambda M: M.incompressible_flow(1 / 180, constant_forcing=1)),
     # odd physical sizes (k = 1 → quadratic size 6; k = 2 → 8) and a single resolved mode
     "tiny": dict(res=(3, 5, 4), dims=(1.0, 1.0, 1.0), lower="SmoothWall", upper="FreeSlipBoundary",
                  procs=lambda M: M.incompressible_flow(0.1)),
@@ -150,7 +158,8 @@ def test_rates_nonhermitian_input(name):
         assert relerr(got[f], rates[f]) < RTOL_RHS64, (name, f)
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "tiny", "channel_flux"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "stretched", "tiny", "channel_flux", "les_min",
+                                  "channel_min"])
 def test_projection_fp64(name):
     ref, gpu = pair(name, seed=5)
     O.apply_projections(ref, ref.state)
@@ -162,7 +171,8 @@ def test_projection_fp64(name):
 
 @pytest.mark.parametrize("name,steps,dt", [("channel", 5, 1e-3), ("les", 5, 5e-4), ("stretched", 4, 1e-3),
                                            ("channel_flux", 5, 1e-3),
-                                           ("les_dealiased", 3, 1e-3), ("tiny", 6, 1e-2), ("custom", 4, 1e-3)])
+                                           ("les_dealiased", 3, 1e-3), ("tiny", 6, 1e-2), ("custom", 4, 1e-3),
+                                           ("les_min", 4, 5e-4), ("channel_min", 4, 1e-3)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_evolve_fp64(name, steps, dt, graph):
     ref, gpu = pair(name, seed=6)
@@ -190,7 +200,7 @@ def test_physical_transfer(name):
         assert relerr(gpu.state[f], exp) < 1e-13
 
 
-@pytest.mark.parametrize("name", ["channel", "les", "custom", "les_square", "les_64", "les_96"])
+@pytest.mark.parametrize("name", ["channel", "les", "custom", "les_square", "les_64", "les_96", "les_min"])
 def test_rates_fp32_mixed(name):
     ref, gpu = pair(name, precision="fp32", seed=9)
     rates = {f: np.zeros_like(a) for f, a in ref.state.items()}
