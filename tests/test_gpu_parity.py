@@ -183,6 +183,30 @@ def test_evolve_fp64(name, steps, dt, graph):
         assert relerr(got[f], ref.state[f]) < RTOL64, (name, f, relerr(got[f], ref.state[f]))
 
 
+@pytest.mark.parametrize("n3,precision", [(100, "fp64"), (300, "fp64"), (1100, "fp64"), (300, "fp32"),
+                                          (1100, "fp32")])
+def test_tall_columns(n3, precision):
+    """Columns past the 64 levels of the default stage chunking: chunks of 8
+    and 16 levels (the float64 (0,0) column pass of the mixed form in chunks
+    of 2 and 4), and beyond 1024 levels the sequential stage kernel; a closed
+    channel, 2 SSPRK33 steps against the oracle."""
+    def build(M, **kw):
+        dom = M.Domain((2 * np.pi, np.pi, 2.0), M.SmoothWall(), M.SmoothWall())
+        return M.Model((4, 6, n3), dom, M.incompressible_flow(1 / 180, constant_forcing=1), **kw)
+    ref, gpu = build(O), build(B, precision=precision)
+    st = random_state(ref, np.random.default_rng(n3))
+    for f in ref.fields:
+        ref.state[f][...] = st[f]
+    gpu.set_state(**st)
+    dt = 1e-4
+    O.evolve(ref, 2 * dt, dt)
+    B.evolve(gpu, 2 * dt, dt)
+    got = gpu.state
+    tol = RTOL64 if precision == "fp64" else 1e-5
+    for f in ref.fields:
+        assert relerr(got[f], ref.state[f]) < tol, (f, relerr(got[f], ref.state[f]))
+
+
 @pytest.mark.parametrize("name", ["channel", "les", "tiny"])
 def test_physical_transfer(name):
     ref, gpu = pair(name, seed=7)
